@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Throughput of the transient path tracer on one MI355X.
+
+A step is one full render pass of the workload (every path of every pixel,
+every bounce, the film accumulation and its compensated merge) with the scene
+and film already resident in HBM. Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--workload icosphere6|box|media]
+
+The CPU baseline (the oracle, oracle/oracle.c, same RNG scheme and workload)
+runs first, on a bounded sample of rows, before anything touches the GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from jarabotransientrendering_amd import scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+# algorithmic HBM bytes per unit of work (wavefront formulation, path state round-trips
+# through HBM between stages):
+BYTES_EXTEND_PER_RAY = 32 + 16          # ray in, hit out
+BYTES_SHADOW_PER_RAY = 32 + 4           # shadow ray in, occlusion out
+BYTES_FILM_PER_CONTRIB = 24             # FP32 RGB read-modify-write of the pass film
+BYTES_SHADE_PER_SEGMENT = 64 + 64 + 20  # path state in + out, NEE record out
+BYTES_PATH_PER_SEGMENT = 300            # whole segment: extension + shadow + state + record + film
+
+
+def workload(name: str):
+    if name == "icosphere6":
+        s = scenes.icosphere_box(level=6, width=256, height=256, time_bins=1024, exposure=0.01, sqrt_spp=32,
+                                 n_incoming=1, max_bounces=8, seed=0)
+        desc = ("81,920-triangle displaced icosphere (subdivision 6, value-noise amplitude 0.05) in the open "
+                "unit box, 0.3x0.3 area light; 256x256 px x 1024 time bins of 0.01; 1024 spp "
+                "(32x32 camera samples x 1 path); max 8 bounces; FP32 film with compensated merge")
+    elif name == "box":
+        s = scenes.cornell_box(width=64, height=64, time_bins=256, exposure=0.01, sqrt_spp=32, n_incoming=1,
+                               max_bounces=4, seed=0)
+        desc = "22-triangle box, point light; 64x64 px x 256 bins of 0.01; 1024 spp; max 4 bounces"
+    elif name == "media":
+        s = scenes.media_box(width=512, height=512, scanline=256, time_bins=1024, exposure=0.01, sqrt_spp=64,
+                             n_incoming=1, max_bounces=16, seed=0)
+        desc = ("box filled with a homogeneous isotropic medium (sigma_s 0.5, sigma_a 0.05), point light; "
+                "scanline 256 of 512x512, 1024 bins of 0.01; 4096 spp; max 16 events")
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return s, desc
+
+
+# ---------------------------------------------------------------------------------------------
+# CPU baseline: the oracle on host cores, same workload scene and RNG scheme, bounded row sample
+
+def _cpu_worker(args):
+    name, rows, sqrt_spp, seed = args
+    from oracle import pyoracle as po
+    s, _ = workload(name)
+    s.render.sqrt_spp = sqrt_spp
+    _, _, st = po.render(s, po.RNG_PER_PATH, rows=rows, seed=seed)
+    return st["paths"], st["render_seconds"]
+
+
+def cpu_baseline(name: str, seconds: float, max_procs: int = 16):
+    """The oracle on `procs` host cores for about `seconds`: each process renders a disjoint row
+    range (or, for a scanline workload, the scanline with its own seed) of the same scene."""
+    from oracle import pyoracle as po
+    if not os.path.exists(po.LIB_PATH):
+        po.build()
+    s, _ = workload(name)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    procs = max(1, min(cores, max_procs))
+    H, W, n_inc = s.film.height, s.film.width, s.render.n_incoming
+    scan = s.film.scanline
+    y_mid = scan if scan >= 0 else H // 2
+    paths1, sec1 = _cpu_worker((name, (y_mid, y_mid + 1), 2, 99))   # calibration
+    rate1 = paths1 / max(sec1, 1e-9)
+    rows_per_proc = 1 if scan >= 0 else max(1, H // procs)
+    spp = max(1, int(round((rate1 * seconds / (rows_per_proc * W * n_inc)) ** 0.5)))
+    jobs = []
+    for p in range(procs):
+        r0 = scan if scan >= 0 else p * rows_per_proc
+        jobs.append((name, (r0, r0 + rows_per_proc), spp, p))
+    ctx = mp.get_context("fork")  # no GPU state exists yet in this process
+    t0 = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    paths = sum(r[0] for r in res)
+    per_core = paths / sum(r[1] for r in res)
+    sample = (f"{procs} processes, each {rows_per_proc} row(s) x {W} px x {spp * spp} camera samples x {n_inc} "
+              f"path(s) of the same scene (per-path RNG streams, as on the GPU): {paths} paths in {wall:.1f} s "
+              f"wall; per core {per_core:.4g} paths/s of render loop")
+    return {"value": paths / wall, "unit": "paths/s", "cores": procs, "kind": "port", "sample": sample,
+            "value_per_core": per_core}
+
+
+# ---------------------------------------------------------------------------------------------
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="icosphere6")
+    ap.add_argument("--wave", type=int, default=0, help="paths per wavefront (0 = library default)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures a single MI355X; --gpus must be 1 (the multi-GPU split is a "
+              "design note, not a measured configuration)", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("RANK", "0")) != 0 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: run as a single process", file=sys.stderr)
+        sys.exit(2)
+
+    scene, desc = workload(args.workload)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.workload, args.cpu_seconds)
+
+    from jarabotransientrendering_amd import renderer, _native as nat
+    world, cam, sampler, film, integ = renderer.setup_scene(scene)
+    eng = renderer.TransientRenderer()
+    eng.wave_paths = args.wave
+    eng.profile = True
+    for k in range(args.warmup):
+        eng.render(None, world, integ, cam, film, sampler, seed=1000 + k)
+    film.clean()
+    steps = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        steps.append(eng.render(None, world, integ, cam, film, sampler, seed=k))
+    wall = time.perf_counter() - t0
+    paths = sum(s["paths"] for s in steps)
+    value = paths / wall
+
+    kern_ms = {n: sum(s["kernel_ms"][n] for s in steps) for n in steps[0]["kernel_ms"]}
+    kern_n = {n: sum(s["kernel_launches"][n] for s in steps) for n in steps[0]["kernel_launches"]}
+    ext = sum(s["extension_rays"] for s in steps)
+    shd = sum(s["shadow_rays"] for s in steps)
+    con = sum(s["contributions"] for s in steps)
+    alg_bytes = {
+        "extend": BYTES_EXTEND_PER_RAY * ext,
+        "shadow": BYTES_SHADOW_PER_RAY * shd + BYTES_FILM_PER_CONTRIB * con,
+        "shade": BYTES_SHADE_PER_SEGMENT * ext,
+        "film_merge": 2 * 4 * film.rows * film.width * film.time_bins * 3 * args.steps,
+        "camera": 80 * paths,
+    }
+    dom = max(kern_ms, key=lambda n: kern_ms[n])
+    avg_ms = kern_ms[dom] / max(kern_n[dom], 1)
+    bytes_per_launch = alg_bytes[dom] / max(kern_n[dom], 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    try:
+        with open(args.traffic_json) as fh:
+            tj = json.load(fh)
+        key = f"{args.workload}:{dom}"
+        if key in tj:
+            traffic = tj[key]["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        traffic = None
+    gpu_time_ms = sum(kern_ms.values())
+    path_gbs = BYTES_PATH_PER_SEGMENT * ext / (wall) / 1e9
+    out = {
+        "metric": "time-resolved paths/s on 1 MI355X",
+        "value": value,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {desc}", "paths_per_step": paths // args.steps,
+                   "triangles": scene.n_tris, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "avg_launch_ms": avg_ms, "launches": kern_n[dom],
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "path_roofline": {"bytes_per_segment": BYTES_PATH_PER_SEGMENT, "segments_per_path": ext / paths,
+                          "achieved": path_gbs, "frac": path_gbs / HBM_PEAK_GBS},
+        "kernels_ms": {k: round(v, 3) for k, v in kern_ms.items()},
+        "kernel_time_share": {k: round(v / gpu_time_ms, 4) for k, v in kern_ms.items()} if gpu_time_ms else None,
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,192 @@
+/*
+ * tr_api.h — C ABI of the MI355X transient path tracer.
+ *
+ * The library replaces the reference renderer's transient path-tracing hot path
+ * (TransientRenderer::render -> World::Li -> BidirectionalPathTracing in
+ * path-tracing-only mode -> StreakCameraFilm::add_samples) with a wavefront
+ * path tracer of hand-written HIP kernels for gfx950.
+ *
+ * Conventions
+ *   - Every entry point returns TR_OK (0) or a negative TR_ERR_* status; no
+ *     exceptions cross the ABI. tr_last_error() returns a message for the last
+ *     failure on the calling thread.
+ *   - Host arrays passed in are caller-owned and copied. Device memory is owned
+ *     by the scene / film handle that allocated it.
+ *   - A handle is used by one host thread at a time; distinct handles may be
+ *     used concurrently. Work is issued on the library's own HIP stream per
+ *     handle and every entry point returns after that work has completed,
+ *     except where noted.
+ *   - The library never falls back to CPU code: without a usable gfx950 device,
+ *     tr_scene_create / tr_film_create fail with TR_ERR_DEVICE.
+ *
+ * Reference interfaces each entry point replaces are cited as file:line in the
+ * reference repository (src/...).
+ */
+#ifndef TR_API_H
+#define TR_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TR_API_VERSION 1
+
+enum {
+    TR_OK = 0,
+    TR_ERR_INVALID = -1,   /* bad argument / handle state */
+    TR_ERR_DEVICE = -2,    /* HIP runtime or kernel failure, or no gfx950 device */
+    TR_ERR_NOMEM = -3,     /* device or host allocation failed */
+    TR_ERR_STATE = -4      /* call order violated (e.g. render before commit) */
+};
+
+typedef struct tr_scene_s* tr_scene;
+typedef struct tr_film_s* tr_film;
+
+/* Lambertian material (UberMaterial + one Lambertian BSDF: bunnykiller.cpp:328-336,
+ * src/Material/UberMaterial.h:71-104, src/Material/Reflectance/Lambertian.h:60-122). */
+typedef struct {
+    float kd[3];
+} tr_material;
+
+/* ---- scene (World) ----------------------------------------------------------------------- */
+
+/* World::World (src/RayTracing/World.h:137-157). */
+int tr_scene_create(tr_scene* out);
+int tr_scene_destroy(tr_scene scene);
+
+/* World::add_triangle_mesh (src/RayTracing/World.h:209-541). Triangles keep the reference's
+ * conventions: geometric/shading normal normalize(cross(p2-p0, p1-p0)) (World.h:449);
+ * geom_id = order of mesh insertion, prim_id = triangle order within the mesh.
+ * xyz: nv*3 floats, idx: nt*3 vertex indices. */
+int tr_scene_add_mesh(tr_scene scene, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                      const tr_material* material, uint32_t* geom_id);
+
+/* PointLightSource (src/LightSource/PointLightSource.h:39-171; CLI bunnykiller.cpp:1194-1207). */
+int tr_scene_add_point_light(tr_scene scene, const float position[3], const float intensity[3]);
+
+/* RectangularAreaLightSource with the CLI parameterisation (bunnykiller.cpp:1242-1261):
+ * corner, a target point (normal = normalize(target - corner)), side lengths, scalar intensity.
+ * (src/LightSource/RectangularAreaLightSource.h:66-80, 222-248). */
+int tr_scene_add_rect_light(tr_scene scene, const float corner[3], const float target[3],
+                            const float size[2], float intensity);
+
+/* World::set_medium with HomogeneousMedium<Isotropic> (src/RayTracing/World.h:711-715,
+ * src/Media/HomogeneousMedium.h:45-127, bunnykiller.cpp:178-228). No time delay. */
+int tr_scene_set_medium(tr_scene scene, const float sigma_a[3], const float sigma_s[3]);
+
+/* World::freeze (src/RayTracing/World.h:543-550): builds the BVH and uploads the scene. */
+int tr_scene_commit(tr_scene scene);
+
+typedef struct {
+    uint32_t n_tris, n_bvh_nodes, bvh_depth, n_lights;
+    uint64_t device_bytes;
+    double build_ms;
+} tr_scene_info;
+int tr_scene_info_get(tr_scene scene, tr_scene_info* info);
+
+/* ---- ray queries (World::first_intersection / intersects) -------------------------------- */
+
+typedef struct {
+    float org[3];
+    float tnear;
+    float dir[3];
+    float tfar;
+} tr_ray; /* 32 bytes */
+
+typedef struct {
+    float t;          /* hit distance, or -1 on a miss */
+    uint32_t geom_id; /* 0xFFFFFFFF on a miss (RTC_INVALID_GEOMETRY_ID) */
+    uint32_t prim_id;
+    uint32_t tri;     /* scene-wide triangle index */
+} tr_hit; /* 16 bytes */
+
+/* Closest hit in (tnear, tfar] for each ray — World::first_intersection's query
+ * (src/RayTracing/World.h:566-615 -> rtcIntersect). Watertight ray/triangle test.
+ * Host pointers; n may be 0. */
+int tr_intersect_batch(tr_scene scene, const tr_ray* rays, uint32_t n, tr_hit* hits);
+/* Same, on device pointers, asynchronously on `stream` (a hipStream_t; NULL = library stream). */
+int tr_intersect_batch_device(tr_scene scene, const tr_ray* d_rays, uint32_t n, tr_hit* d_hits,
+                              void* stream);
+
+/* ---- film (StreakCameraFilm) ------------------------------------------------------------- */
+
+typedef struct {
+    int32_t width, height, time_bins;
+    float exposure;       /* -film-exposure */
+    float offset;         /* -film-offset (StreakCameraFilm::set_offset) */
+    int32_t scanline;     /* -film-scanline y, or -1 for the whole film */
+    int32_t camera_unwarp;/* -camera-unwarp */
+} tr_film_desc;
+
+/* StreakCameraFilm::StreakCameraFilm (src/Film/StreakCameraFilm.h:91-121). Device buffers:
+ * per-pass accumulation plus a Kahan/Neumaier-compensated FP32 sum, rows x W x T x 3 floats each
+ * (rows = 1 in scanline mode). */
+int tr_film_create(tr_film* out, const tr_film_desc* desc);
+int tr_film_destroy(tr_film film);
+/* Film::clean: zero all sums and weights. */
+int tr_film_clear(tr_film film);
+/* Normalised streak data, StreakCameraFilm::average_and_save_slice (StreakCameraFilm.h:357-397)
+ * with every pixel divided by its own weight (0/0 -> 0): out[row][x][t][c], rows x W x T x 3.
+ * steady (optional, may be NULL): the time-integrated image out[row][x][c] (Film::draw_pixel). */
+int tr_film_read(tr_film film, float* out, float* steady);
+/* Un-normalised compensated sums (sum + compensation) as double, and per-pixel weights. */
+int tr_film_read_sums(tr_film film, double* sums, double* weights);
+/* Device pointers of the compensated-sum buffers (read-only use by the caller). */
+int tr_film_device_buffers(tr_film film, float** d_sum, float** d_comp);
+
+/* ---- rendering (TransientRenderer::render) ------------------------------------------------ */
+
+typedef struct {
+    float position[3];
+    float focus[3];
+    float up[3];
+    float fov_deg;        /* -camera-fov: view-plane distance 1/tan(fov/2) (bunnykiller.cpp:1156-1160) */
+} tr_camera;
+
+typedef struct {
+    int32_t sqrt_spp;     /* -camera-spp: sqrt of camera samples per pixel (StratifiedSampler) */
+    int32_t n_incoming;   /* -bidirectional-path-tracing N: paths per camera sample */
+    int32_t max_bounces;  /* -max-nb-bounces */
+    int32_t pad0;
+    uint64_t seed;        /* -new-seed */
+    uint32_t wave_paths;  /* paths in flight per wavefront (0 = library default) */
+    uint32_t flags;       /* TR_RENDER_* */
+} tr_render_desc;
+
+enum {
+    TR_RENDER_PROFILE = 1u  /* time each kernel with HIP events (tr_render_stats.kernel_ms) */
+};
+
+enum {
+    TR_K_CAMERA = 0, TR_K_EXTEND = 1, TR_K_SHADE = 2, TR_K_SHADOW = 3, TR_K_FILM_MERGE = 4,
+    TR_K_COUNT = 8
+};
+
+typedef struct {
+    uint64_t paths, extension_rays, shadow_rays, contributions;
+    uint32_t waves, bounces_max;
+    double render_ms;              /* host wall time of the whole call */
+    double kernel_ms[TR_K_COUNT];  /* summed HIP-event time per kernel kind (TR_RENDER_PROFILE) */
+    uint32_t kernel_launches[TR_K_COUNT];
+} tr_render_stats;
+
+/* TransientRenderer::render (src/Renderer/TransientRenderer.h:77-201) driving
+ * BidirectionalPathTracing::operator()(ray, samples, ...) in path-tracing-only SteadyState mode
+ * (src/Integrator/BidirectionalPathTracing.h:1100-1140) and StreakCameraFilm::add_samples
+ * (src/Film/StreakCameraFilm.h:139-192). Accumulates into `film` (successive calls are passes
+ * merged with compensated summation). Random numbers: one PCG32 stream per camera sample,
+ * per path and per path for light sampling, all seeded from desc->seed. */
+int tr_render_transient(tr_scene scene, const tr_camera* camera, tr_film film,
+                        const tr_render_desc* desc, tr_render_stats* stats);
+
+/* ---- misc --------------------------------------------------------------------------------- */
+
+const char* tr_last_error(void);
+int tr_device_info(char* name, int name_len, int* compute_units, uint64_t* global_mem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TR_API_H */

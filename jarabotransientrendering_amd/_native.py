@@ -1,0 +1,150 @@
+"""ctypes binding of the C ABI in include/tr_api.h (libtransient_hip.so).
+
+The library is loaded from the package's _lib/ directory. There is no fallback:
+if the library is missing or cannot find a gfx950 device, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .build import LIB
+
+TR_OK = 0
+TR_ERR_INVALID = -1
+TR_ERR_DEVICE = -2
+TR_ERR_NOMEM = -3
+TR_ERR_STATE = -4
+
+TR_RENDER_PROFILE = 1
+K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FILM_MERGE = 0, 1, 2, 3, 4
+K_NAMES = {K_CAMERA: "camera", K_EXTEND: "extend", K_SHADE: "shade", K_SHADOW: "shadow",
+           K_FILM_MERGE: "film_merge"}
+TR_K_COUNT = 8
+
+
+class TrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("kd", ctypes.c_float * 3)]
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("n_tris", ctypes.c_uint32), ("n_bvh_nodes", ctypes.c_uint32), ("bvh_depth", ctypes.c_uint32),
+                ("n_lights", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64), ("build_ms", ctypes.c_double)]
+
+
+class Ray(ctypes.Structure):
+    _fields_ = [("org", ctypes.c_float * 3), ("tnear", ctypes.c_float), ("dir", ctypes.c_float * 3),
+                ("tfar", ctypes.c_float)]
+
+
+class Hit(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_float), ("geom_id", ctypes.c_uint32), ("prim_id", ctypes.c_uint32),
+                ("tri", ctypes.c_uint32)]
+
+
+class FilmDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("time_bins", ctypes.c_int32),
+                ("exposure", ctypes.c_float), ("offset", ctypes.c_float), ("scanline", ctypes.c_int32),
+                ("camera_unwarp", ctypes.c_int32)]
+
+
+class CameraDesc(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_float * 3), ("focus", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("fov_deg", ctypes.c_float)]
+
+
+class RenderDesc(ctypes.Structure):
+    _fields_ = [("sqrt_spp", ctypes.c_int32), ("n_incoming", ctypes.c_int32), ("max_bounces", ctypes.c_int32),
+                ("pad0", ctypes.c_int32), ("seed", ctypes.c_uint64), ("wave_paths", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("paths", ctypes.c_uint64), ("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+                ("contributions", ctypes.c_uint64), ("waves", ctypes.c_uint32), ("bounces_max", ctypes.c_uint32),
+                ("render_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double * TR_K_COUNT),
+                ("kernel_launches", ctypes.c_uint32 * TR_K_COUNT)]
+
+    def as_dict(self):
+        d = {k: int(getattr(self, k)) for k in ("paths", "extension_rays", "shadow_rays", "contributions", "waves",
+                                                 "bounces_max")}
+        d["render_ms"] = float(self.render_ms)
+        d["kernel_ms"] = {K_NAMES[k]: float(self.kernel_ms[k]) for k in K_NAMES}
+        d["kernel_launches"] = {K_NAMES[k]: int(self.kernel_launches[k]) for k in K_NAMES}
+        return d
+
+
+# exported symbol -> (restype, argtypes); the header declares exactly these
+SIGNATURES = {
+    "tr_scene_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "tr_scene_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "tr_scene_add_mesh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                         ctypes.c_uint32, ctypes.POINTER(Material), ctypes.POINTER(ctypes.c_uint32)]),
+    "tr_scene_add_point_light": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float * 3, ctypes.c_float * 3]),
+    "tr_scene_add_rect_light": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float * 3, ctypes.c_float * 3,
+                                               ctypes.c_float * 2, ctypes.c_float]),
+    "tr_scene_set_medium": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "tr_scene_commit": (ctypes.c_int, [ctypes.c_void_p]),
+    "tr_scene_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SceneInfo)]),
+    "tr_intersect_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "tr_intersect_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                 ctypes.c_void_p, ctypes.c_void_p]),
+    "tr_film_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(FilmDesc)]),
+    "tr_film_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "tr_film_clear": (ctypes.c_int, [ctypes.c_void_p]),
+    "tr_film_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "tr_film_read_sums": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "tr_film_device_buffers": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                              ctypes.POINTER(ctypes.c_void_p)]),
+    "tr_render_transient": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CameraDesc), ctypes.c_void_p,
+                                           ctypes.POINTER(RenderDesc), ctypes.POINTER(RenderStats)]),
+    "tr_last_error": (ctypes.c_char_p, []),
+    "tr_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_uint64)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"HIP library not built: {LIB} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != TR_OK:
+        raise TrError(rc, lib().tr_last_error().decode(errors="replace"))
+
+
+def f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cu = ctypes.c_int()
+    mem = ctypes.c_uint64()
+    check(lib().tr_device_info(name, 256, ctypes.byref(cu), ctypes.byref(mem)))
+    return {"name": name.value.decode(), "compute_units": cu.value, "global_mem": mem.value}
+
+
+def as_ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)

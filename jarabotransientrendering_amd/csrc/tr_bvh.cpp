@@ -1,0 +1,234 @@
+// Host-side BVH builder: binned surface-area-heuristic split (16 bins per axis) over triangle
+// centroids, leaves of at most kLeafMax triangles, depth capped at kBvhMaxDepth by switching to
+// object-median splits below depth 32. Replaces the BVH that Embree builds in rtcCommit
+// (World::freeze, src/RayTracing/World.h:543-550) for the GPU traversal kernels.
+#include "tr_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace {
+
+struct Box {
+    float lo[3], hi[3];
+    void reset()
+    {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::numeric_limits<float>::infinity();
+            hi[k] = -std::numeric_limits<float>::infinity();
+        }
+    }
+    void grow(const Box& o)
+    {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], o.lo[k]);
+            hi[k] = std::max(hi[k], o.hi[k]);
+        }
+    }
+    void grow(const float* p)
+    {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+        }
+    }
+    float area() const
+    {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0.f)) return 0.f;
+        return 2.f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct Builder {
+    std::vector<Box> pbox;
+    std::vector<float> pcen;  // 3 per prim
+    std::vector<uint32_t> prim;
+    std::vector<BvhNodeHost> nodes;
+    int max_depth = 0;
+
+    Box bounds(uint32_t first, uint32_t count, Box* centroid) const
+    {
+        Box b, c;
+        b.reset();
+        c.reset();
+        for (uint32_t i = first; i < first + count; ++i) {
+            b.grow(pbox[prim[i]]);
+            c.grow(&pcen[3 * prim[i]]);
+        }
+        *centroid = c;
+        return b;
+    }
+
+    // Partition prim[first, first+count) and return the split position (first < mid < first+count).
+    uint32_t split(uint32_t first, uint32_t count, const Box& cb, int depth)
+    {
+        constexpr int NB = 16;
+        int best_axis = -1, best_bin = 0;
+        float best_cost = std::numeric_limits<float>::infinity();
+        if (depth < 32) {
+            for (int ax = 0; ax < 3; ++ax) {
+                float ext = cb.hi[ax] - cb.lo[ax];
+                if (!(ext > 0.f)) continue;
+                Box bins[NB];
+                uint32_t cnt[NB] = {};
+                for (auto& b : bins) b.reset();
+                float scale = NB / ext;
+                for (uint32_t i = first; i < first + count; ++i) {
+                    uint32_t p = prim[i];
+                    int k = std::min(NB - 1, std::max(0, int((pcen[3 * p + ax] - cb.lo[ax]) * scale)));
+                    bins[k].grow(pbox[p]);
+                    cnt[k]++;
+                }
+                float la[NB];
+                uint32_t lc[NB];
+                Box acc;
+                acc.reset();
+                uint32_t c = 0;
+                for (int k = 0; k < NB; ++k) {
+                    acc.grow(bins[k]);
+                    c += cnt[k];
+                    la[k] = acc.area();
+                    lc[k] = c;
+                }
+                acc.reset();
+                c = 0;
+                for (int k = NB - 1; k > 0; --k) {
+                    acc.grow(bins[k]);
+                    c += cnt[k];
+                    if (lc[k - 1] == 0 || c == 0) continue;
+                    float cost = la[k - 1] * float(lc[k - 1]) + acc.area() * float(c);
+                    if (cost < best_cost) {
+                        best_cost = cost;
+                        best_axis = ax;
+                        best_bin = k;
+                    }
+                }
+            }
+        }
+        if (best_axis >= 0) {
+            float ext = cb.hi[best_axis] - cb.lo[best_axis];
+            float scale = NB / ext;
+            auto mid = std::partition(prim.begin() + first, prim.begin() + first + count, [&](uint32_t p) {
+                int k = std::min(NB - 1, std::max(0, int((pcen[3 * p + best_axis] - cb.lo[best_axis]) * scale)));
+                return k < best_bin;
+            });
+            uint32_t m = uint32_t(mid - prim.begin());
+            if (m > first && m < first + count) return m;
+        }
+        // object median along the widest centroid axis (also bounds the depth)
+        int ax = 0;
+        float ext = -1.f;
+        for (int k = 0; k < 3; ++k)
+            if (cb.hi[k] - cb.lo[k] > ext) {
+                ext = cb.hi[k] - cb.lo[k];
+                ax = k;
+            }
+        uint32_t m = first + count / 2;
+        std::nth_element(prim.begin() + first, prim.begin() + m, prim.begin() + first + count,
+                         [&](uint32_t x, uint32_t y) {
+                             float cx = pcen[3 * x + ax], cy = pcen[3 * y + ax];
+                             return cx < cy || (cx == cy && x < y);
+                         });
+        return m;
+    }
+
+    // Fill child slot `side` of node `ni` with the subtree over prim[first, first+count).
+    void child(int32_t ni, int side, uint32_t first, uint32_t count, int depth)
+    {
+        Box cb;
+        Box b = bounds(first, count, &cb);
+        BvhNodeHost& n = nodes[ni];
+        float* xy = side == 0 ? n.a : n.b;
+        xy[0] = b.lo[0];
+        xy[1] = b.hi[0];
+        xy[2] = b.lo[1];
+        xy[3] = b.hi[1];
+        n.c[side * 2 + 0] = b.lo[2];
+        n.c[side * 2 + 1] = b.hi[2];
+        max_depth = std::max(max_depth, depth);
+        if (count <= uint32_t(kLeafMax)) {
+            nodes[ni].d[side] = int32_t(first);
+            nodes[ni].d[2 + side] = int32_t(count);
+            return;
+        }
+        int32_t ci = int32_t(nodes.size());
+        nodes.emplace_back();
+        nodes[ni].d[side] = ci;
+        nodes[ni].d[2 + side] = 0;
+        uint32_t mid = split(first, count, cb, depth);
+        child(ci, 0, first, mid - first, depth + 1);
+        child(ci, 1, mid, first + count - mid, depth + 1);
+    }
+};
+
+void empty_child(BvhNodeHost& n, int side)
+{
+    const float inf = std::numeric_limits<float>::infinity();
+    float* xy = side == 0 ? n.a : n.b;
+    xy[0] = inf;
+    xy[1] = -inf;
+    xy[2] = inf;
+    xy[3] = -inf;
+    n.c[side * 2 + 0] = inf;
+    n.c[side * 2 + 1] = -inf;
+    n.d[side] = 0;
+    n.d[2 + side] = -1;  // empty: the traversal skips children with a negative count
+}
+
+}  // namespace
+
+BvhBuildResult build_bvh(const float* tris, const int32_t* tri_mat, uint32_t n)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    Builder B;
+    B.pbox.resize(n);
+    B.pcen.resize(3 * size_t(n));
+    B.prim.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* t = tris + 9 * size_t(i);
+        B.pbox[i].reset();
+        B.pbox[i].grow(t);
+        B.pbox[i].grow(t + 3);
+        B.pbox[i].grow(t + 6);
+        for (int k = 0; k < 3; ++k) B.pcen[3 * i + k] = 0.5f * (B.pbox[i].lo[k] + B.pbox[i].hi[k]);
+        B.prim[i] = i;
+    }
+    B.nodes.reserve(n > 0 ? 2 * size_t(n) / 3 + 4 : 4);
+    B.nodes.emplace_back();
+    std::memset(&B.nodes[0], 0, sizeof(BvhNodeHost));
+    if (n == 0) {
+        empty_child(B.nodes[0], 0);
+        empty_child(B.nodes[0], 1);
+    } else if (n <= uint32_t(kLeafMax)) {
+        B.child(0, 0, 0, n, 1);
+        empty_child(B.nodes[0], 1);
+    } else {
+        Box cb;
+        B.bounds(0, n, &cb);
+        uint32_t mid = B.split(0, n, cb, 0);
+        B.child(0, 0, 0, mid, 1);
+        B.child(0, 1, mid, n - mid, 1);
+    }
+    BvhBuildResult r;
+    r.nodes = std::move(B.nodes);
+    r.depth = B.max_depth;
+    r.tris.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t p = B.prim[i];
+        const float* t = tris + 9 * size_t(p);
+        TriHost& o = r.tris[i];
+        int32_t id = int32_t(p), m = tri_mat[p];
+        std::memcpy(o.p0, t, 12);
+        std::memcpy(&o.p0[3], &id, 4);
+        std::memcpy(o.p1, t + 3, 12);
+        std::memcpy(&o.p1[3], &m, 4);
+        std::memcpy(o.p2, t + 6, 12);
+        o.p2[3] = 0.f;
+    }
+    r.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return r;
+}

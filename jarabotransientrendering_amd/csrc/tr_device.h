@@ -1,0 +1,208 @@
+// Device-side building blocks for gfx950: PCG32 streams, watertight BVH traversal, light and
+// BSDF sampling. Included only by HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "tr_internal.h"
+
+namespace trd {
+
+// ---- float3 helpers ------------------------------------------------------------------------
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 mk(float4 v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 mul(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b)
+{
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ f3 normalize(f3 a)
+{
+    float inv = 1.0f / sqrtf(dot(a, a));
+    return a * inv;
+}
+__device__ __forceinline__ float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// ---- PCG32 (the reference's Random::RNG, src/Utils/RandomNumbers.h:119-174, 83-98), one stream
+// per (seed, sequence). The increment is (seq << 1) | 1 and is recomputed from the sequence id. --
+__device__ __forceinline__ uint32_t pcg_next(uint64_t& state, uint64_t inc)
+{
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    uint32_t xorshifted = uint32_t(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = uint32_t(old >> 59u);
+    return (xorshifted >> rot) | (xorshifted << ((0u - rot) & 31u));
+}
+__device__ __forceinline__ float pcg_float(uint64_t& state, uint64_t inc)
+{
+    uint32_t bits = (pcg_next(state, inc) >> 9) | 0x3f800000u;
+    return __uint_as_float(bits) - 1.0f;
+}
+__device__ __forceinline__ uint64_t pcg_inc(uint64_t seq) { return (seq << 1u) | 1u; }
+__device__ __forceinline__ uint64_t pcg_seed(uint64_t initstate, uint64_t seq)
+{
+    uint64_t inc = pcg_inc(seq), s = 0u;
+    pcg_next(s, inc);
+    s += initstate;
+    pcg_next(s, inc);
+    return s;
+}
+
+// ---- wave-aggregated queue append (one atomic per wavefront) -------------------------------
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
+{
+    uint64_t mask = __ballot(pred);
+    uint32_t lane = __lane_id();
+    uint32_t base = 0;
+    if (mask != 0) {
+        int leader = __ffsll((long long)mask) - 1;
+        if (int(lane) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+        base = __shfl(base, leader);
+    }
+    return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+// ---- ray/box and watertight ray/triangle -----------------------------------------------------
+struct RayPre {
+    f3 o, d, inv;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+// reciprocal for slab tests; zero components become +-1e20 so that (lo - o) * inv never
+// evaluates 0 * inf (a ray lying in a box face plane stays inside that slab)
+__device__ __forceinline__ float safe_rcp(float d)
+{
+    return 1.0f / (fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d);
+}
+
+__device__ __forceinline__ RayPre ray_pre(f3 o, f3 d)
+{
+    RayPre r;
+    r.o = o;
+    r.d = d;
+    r.inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    if (comp(d, kz) < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.Sz = 1.0f / comp(d, kz);
+    r.Sx = comp(d, kx) * r.Sz;
+    r.Sy = comp(d, ky) * r.Sz;
+    return r;
+}
+
+// Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1) 2013, Sec. 4 (no
+// backface culling; double-precision re-evaluation when an edge function is exactly zero).
+// Returns the distance, or -1 when the ray misses the triangle's interior/edges.
+__device__ __forceinline__ float tri_watertight(const RayPre& r, f3 p0, f3 p1, f3 p2)
+{
+    f3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
+    float Akx = comp(A, r.kx), Aky = comp(A, r.ky), Akz = comp(A, r.kz);
+    float Bkx = comp(B, r.kx), Bky = comp(B, r.ky), Bkz = comp(B, r.kz);
+    float Ckx = comp(C, r.kx), Cky = comp(C, r.ky), Ckz = comp(C, r.kz);
+    float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
+    float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
+    float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return -1.0f;
+    float det = U + V + W;
+    if (det == 0.0f) return -1.0f;
+    float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+    return T / det;
+}
+
+// Closest (ANY = false) or any (ANY = true) hit in (tnear, tfar]. On a closest hit `tfar` is
+// updated. Equal-distance ties resolve to the lower scene triangle index. `stack` points at this
+// thread's first LDS stack slot; consecutive slots are `stride` ints apart.
+// Returns the scene triangle index or -1.
+template <bool ANY>
+__device__ __forceinline__ int32_t trace(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                         const RayPre& r, float tnear, float& tfar, int32_t* stack,
+                                         int stride)
+{
+    int32_t hit = -1;
+    int32_t node = 0;
+    int sp = 0;
+    for (;;) {
+        const float4* n = nodes + 4 * size_t(node);
+        float4 a = n[0], b = n[1], c = n[2];
+        int4 dd = *reinterpret_cast<const int4*>(n + 3);
+        // slab tests as (lo - o) * inv: exact at a face through the origin, three roundings per
+        // bound, so the far distance is widened by 1 + 2*gamma(3) (PBRT 3.9.2)
+        float lx0 = (a.x - r.o.x) * r.inv.x, lx1 = (a.y - r.o.x) * r.inv.x;
+        float ly0 = (a.z - r.o.y) * r.inv.y, ly1 = (a.w - r.o.y) * r.inv.y;
+        float lz0 = (c.x - r.o.z) * r.inv.z, lz1 = (c.y - r.o.z) * r.inv.z;
+        float rx0 = (b.x - r.o.x) * r.inv.x, rx1 = (b.y - r.o.x) * r.inv.x;
+        float ry0 = (b.z - r.o.y) * r.inv.y, ry1 = (b.w - r.o.y) * r.inv.y;
+        float rz0 = (c.z - r.o.z) * r.inv.z, rz1 = (c.w - r.o.z) * r.inv.z;
+        float tl0 = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), tnear));
+        float tl1 = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), tfar)) * 1.00000036f;
+        float tr0 = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), tnear));
+        float tr1 = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), tfar)) * 1.00000036f;
+        bool hl = (tl0 <= tl1) && dd.z >= 0;
+        bool hr = (tr0 <= tr1) && dd.w >= 0;
+        // leaf children: test their triangles now
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            bool h = side == 0 ? hl : hr;
+            int cnt = side == 0 ? dd.z : dd.w;
+            if (h && cnt > 0) {
+                int first = side == 0 ? dd.x : dd.y;
+                for (int i = first; i < first + cnt; ++i) {
+                    const float4* t = tris + 3 * size_t(i);
+                    float4 q0 = t[0], q1 = t[1], q2 = t[2];
+                    float th = tri_watertight(r, mk(q0), mk(q1), mk(q2));
+                    int32_t id = __float_as_int(q0.w);
+                    if (th > tnear && th <= tfar && (th < tfar || hit < 0 || id < hit)) {
+                        hit = id;
+                        if (ANY) return hit;
+                        tfar = th;
+                    }
+                }
+                if (side == 0) hl = false;
+                else hr = false;
+            }
+        }
+        if (hl && hr) {
+            bool left_first = tl0 <= tr0;
+            stack[sp * stride] = left_first ? dd.y : dd.x;
+            ++sp;
+            node = left_first ? dd.x : dd.y;
+        } else if (hl) {
+            node = dd.x;
+        } else if (hr) {
+            node = dd.y;
+        } else {
+            if (sp == 0) break;
+            --sp;
+            node = stack[sp * stride];
+        }
+    }
+    return hit;
+}
+
+}  // namespace trd

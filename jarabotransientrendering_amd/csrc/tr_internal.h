@@ -1,0 +1,51 @@
+// Internal layouts shared by the host BVH builder, the host API and the HIP kernels.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+// ---- BVH (binary, both child boxes stored in the parent; 64 bytes per node) ----------------
+//   a = (L.lo.x, L.hi.x, L.lo.y, L.hi.y)
+//   b = (R.lo.x, R.hi.x, R.lo.y, R.hi.y)
+//   c = (L.lo.z, L.hi.z, R.lo.z, R.hi.z)
+//   d = (L.index, R.index, L.count, R.count); count > 0: leaf, index = first entry in the
+//       leaf-ordered triangle array; count == 0: inner node index; count < 0: empty child
+//       (only in a root over fewer than two leaves), never entered.
+struct alignas(16) BvhNodeHost {
+    float a[4], b[4], c[4];
+    int32_t d[4];
+};
+static_assert(sizeof(BvhNodeHost) == 64, "node layout");
+
+// Leaf-ordered triangle record: 3 x float4 = 48 bytes.
+//   p0.xyz, w = scene triangle index (as int bits); p1.xyz, w = material index; p2.xyz, w = 0
+struct alignas(16) TriHost {
+    float p0[4], p1[4], p2[4];
+};
+static_assert(sizeof(TriHost) == 48, "tri layout");
+
+constexpr int kBvhMaxDepth = 48;   // builder guarantee; traversal stacks hold kBvhStack entries
+constexpr int kBvhStack = 64;
+constexpr int kLeafMax = 4;
+
+struct BvhBuildResult {
+    std::vector<BvhNodeHost> nodes;  // root = node 0 (always an inner node)
+    std::vector<TriHost> tris;       // leaf order
+    int depth = 0;
+    double build_ms = 0.0;
+};
+
+// tris: n*9 floats (p0, p1, p2 as given), tri_mat: n material indices.
+BvhBuildResult build_bvh(const float* tris, const int32_t* tri_mat, uint32_t n);
+
+// ---- lights ------------------------------------------------------------------------------
+enum : int32_t { kLightPoint = 0, kLightRect = 1 };
+struct alignas(16) LightHost {
+    float pos[4];     // point: position; rect: corner
+    float left[4];    // rect: m_left * size_x
+    float up[4];      // rect: m_up * size_y
+    float normal[4];  // rect: normal; w = 1/area
+    float I[4];       // RGB intensity (rect: scalar replicated), w = type (as float)
+};
+static_assert(sizeof(LightHost) == 80, "light layout");

@@ -1,0 +1,1158 @@
+// Wavefront transient path tracer for gfx950 (MI355X) and the C ABI declared in include/tr_api.h.
+//
+// Per wavefront of `wave` paths:
+//   camera_kernel                 camera ray + per-path PCG32 streams       (TransientRenderer.h:130-163,
+//                                                                            StratifiedSampler.h:47-89,
+//                                                                            PinholeCamera.h:82-108)
+//   for each bounce:
+//     extend_kernel               closest hit of every queued ray           (World::first_intersection)
+//     shade_kernel                free-flight correction, vertex, NEE light
+//                                 sample -> shadow queue, roulette, next
+//                                 direction -> next ray queue               (BidirectionalPathTracing.h:486-692)
+//     shadow_kernel               occlusion of every shadow ray; visible
+//                                 contributions are binned into the pass
+//                                 film with float atomics                   (World::intersects,
+//                                                                            StreakCameraFilm.h:139-192)
+//   film_merge_kernel             pass film -> Neumaier-compensated FP32 film (once per render call)
+//
+// Path state lives in HBM as structure-of-arrays float4/uint4 records indexed by a path slot;
+// queues hold slot indices. Queue counters live in device memory so no host round trip is needed
+// between kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "tr_api.h"
+#include "tr_device.h"
+#include "tr_internal.h"
+
+using namespace trd;
+
+namespace {
+
+constexpr int kTraceBlock = 128;   // traversal kernels: 128 threads, kBvhStack ints of LDS each
+constexpr int kBlock = 256;        // other kernels
+constexpr uint32_t kDefaultWave = 1u << 23;
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr float kInv4Pi = 0.0795774715459476678844f;  // .25 * M_1_PI (IsotropicPhaseFunction.h:43-47)
+constexpr float kSigmaVisibility = 1e-5f;             // _SIGMA_VISIBILITY_ (bunnykiller.h:76-78)
+
+// counters[] layout (uint32 words)
+enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_STAT_EXT = 4, C_STAT_SHADOW = 6, C_STAT_CONTRIB = 8, C_WORDS = 16 };
+
+struct SceneDev {
+    const float4* nodes;
+    const float4* tris;
+    const float4* tri_normal;  // per scene triangle: shading normal xyz, w = material index (int bits)
+    const float4* mats;        // Kd xyz, w = mean albedo used by the roulette
+    const LightHost* lights;
+    int32_t n_lights;
+    int32_t has_medium;
+    float3 sigma_s, sigma_t;
+    float lambda;              // mean extinction (HomogeneousMedium.h:50-51)
+    float medium_albedo;       // mean of sigma_s / sigma_t
+};
+
+struct RenderDev {
+    // film
+    float* film;               // pass buffer, [rows][W][T][3]
+    int32_t W, H, T, row0, spp, spp2, n_inc;
+    float exposure, offset, total_time, inv_inc;
+    int32_t unwarp, max_bounces;
+    // camera
+    float3 cpos, nv, uv, vv;
+    float fd, xw2, yw2, ratio, inv_spp;
+    uint64_t seed;
+};
+
+struct WaveDev {
+    float4* ray_o;    // origin xyz, w = sampled free-flight distance of this segment (inf without medium)
+    float4* ray_d;    // direction xyz, w = 0
+    float4* thr;      // throughput xyz (with the pending BSDF/phase weight and roulette), w = time
+    uint4* misc;      // x = path id lo, y = path id hi, z/w = path-stream PCG32 state
+    uint4* extra;     // x/y = light-stream PCG32 state, z = level, w = unwarp distance (float bits)
+    float2* hit;      // x = distance, y = scene triangle (int bits) or -1
+    uint32_t* q[2];
+    float4* sh_o;     // light point xyz, w = distance
+    float4* sh_d;     // direction xyz, w = film element index (uint bits)
+    float4* sh_c;     // contribution rgb
+    uint32_t* counters;
+};
+
+__device__ __forceinline__ uint64_t u64(uint32_t lo, uint32_t hi) { return uint64_t(lo) | (uint64_t(hi) << 32); }
+
+// ---------------------------------------------------------------------------------------------
+// camera: one thread per path of the wavefront
+__global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S, RenderDev R, WaveDev Wv, uint64_t wave_base,
+                                                        uint32_t n)
+{
+    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot == 0) Wv.counters[C_Q0] = n;
+    if (slot >= n) return;
+    uint64_t pid_local = wave_base + slot;
+    uint64_t cs_local = pid_local / uint64_t(R.n_inc);
+    uint32_t inc_i = uint32_t(pid_local - cs_local * uint64_t(R.n_inc));
+    uint64_t pix_local = cs_local / uint64_t(R.spp2);
+    uint32_t sub = uint32_t(cs_local - pix_local * uint64_t(R.spp2));
+    uint32_t xr = uint32_t(pix_local % uint64_t(R.W));
+    uint32_t y = uint32_t(pix_local / uint64_t(R.W)) + uint32_t(R.row0);
+    uint32_t sx = sub % uint32_t(R.spp), sy = sub / uint32_t(R.spp);
+    uint64_t cs = (uint64_t(y) * uint64_t(R.W) + xr) * uint64_t(R.spp2) + sub;
+    uint64_t pid = cs * uint64_t(R.n_inc) + inc_i;
+
+    // StratifiedSampler::get_next_sample: jittered position clamped inside the pixel
+    uint64_t cinc = pcg_inc((1ull << 62) | cs);
+    uint64_t cst = pcg_seed(R.seed, (1ull << 62) | cs);
+    float jx = pcg_float(cst, cinc), jy = pcg_float(cst, cinc);
+    float px = fminf(float(xr) + (float(sx) + jx) * R.inv_spp, __uint_as_float(__float_as_uint(float(xr + 1)) - 1u));
+    float py = fminf(float(y) + (float(sy) + jy) * R.inv_spp, __uint_as_float(__float_as_uint(float(y + 1)) - 1u));
+    // Film::window_coords2image_coords + PinholeCamera::get_ray_direction
+    float ic0 = (px - R.xw2) / float(R.W);
+    float ic1 = (py - R.yw2) / float(R.H) / R.ratio;
+    f3 nv = mk(R.nv.x, R.nv.y, R.nv.z), uv = mk(R.uv.x, R.uv.y, R.uv.z), vv = mk(R.vv.x, R.vv.y, R.vv.z);
+    f3 dir = nv * R.fd + uv * (2.0f * ic0) - vv * (2.0f * ic1);
+    dir = normalize(normalize(dir));  // get_ray_direction normalises, Ray3D normalises again
+
+    uint64_t pinc = pcg_inc(2 * pid);
+    uint64_t pst = pcg_seed(R.seed, 2 * pid);
+    uint64_t nst = pcg_seed(R.seed, 2 * pid + 1);
+    float sampled = __int_as_float(0x7f800000);
+    if (S.has_medium) {
+        float eps = pcg_float(pst, pinc);
+        sampled = -logf(eps) / S.lambda;
+    }
+    Wv.ray_o[slot] = make_float4(R.cpos.x, R.cpos.y, R.cpos.z, sampled);
+    Wv.ray_d[slot] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    Wv.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    Wv.misc[slot] = make_uint4(uint32_t(pid), uint32_t(pid >> 32), uint32_t(pst), uint32_t(pst >> 32));
+    Wv.extra[slot] = make_uint4(uint32_t(nst), uint32_t(nst >> 32), 0u, __float_as_uint(3.402823466e+38f));
+    Wv.q[0][slot] = slot;
+}
+
+// ---------------------------------------------------------------------------------------------
+// extend: closest hit for each queued ray
+__global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S, WaveDev Wv, int cur)
+{
+    __shared__ int32_t stack[kBvhStack * kTraceBlock];
+    const uint32_t count = Wv.counters[cur];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0) {
+        Wv.counters[cur ^ 1] = 0;
+        Wv.counters[C_SHADOW] = 0;
+        atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_EXT), (unsigned long long)count);
+    }
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = gid; i < count; i += stride) {
+        uint32_t slot = Wv.q[cur][i];
+        float4 o = Wv.ray_o[slot], d = Wv.ray_d[slot];
+        RayPre r = ray_pre(mk(o), mk(d));
+        float tfar = __int_as_float(0x7f800000);
+        int32_t tri = trace<false>(S.nodes, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock);
+        Wv.hit[slot] = make_float2(tri >= 0 ? tfar : -1.0f, __int_as_float(tri));
+    }
+}
+
+// Lambertian cosine-weighted direction in the reference's shading frame
+// (Sampling.h:131-143, Intersection.h:167-178, Lambertian.h:107-122).
+__device__ __forceinline__ f3 sample_lambert(f3 n, float e1, float e2, float* cos_sampled)
+{
+    float phi = 2.0f * kPi * e1;
+    float ct = sqrtf(e2);
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    f3 l = mk(cp * st, ct, sp * st);
+    f3 aux = (fabsf(n.x) > fabsf(n.z)) ? mk(-n.y, n.x, n.z) : mk(0.0f, -n.z, n.y);
+    f3 tv = normalize(aux);
+    f3 tu = cross(n, tv);
+    *cos_sampled = ct;
+    return normalize(tu * l.x + n * l.y + tv * l.z);
+}
+
+// Isotropic phase-function direction: uniform sphere rotated so that +y maps onto wi
+// (Sampling.h:80-94, IsotropicPhaseFunction.h:54-60, VectorN.h:429-523).
+__device__ __forceinline__ f3 sample_isotropic(f3 wi, float e1, float e2)
+{
+    float phi = 2.0f * kPi * e1;
+    float ct = 1.0f - 2.0f * e2;
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    f3 v = mk(cp * st, ct, sp * st);
+    // transform_matrix_to(from = (0,1,0), to = wi)
+    f3 from = mk(0.0f, 1.0f, 0.0f);
+    f3 vx = cross(from, wi);
+    float e = wi.y;
+    float f = fabsf(e);
+    float m[3][3];
+    if (double(f) > 1.0 - double(1.e-4f)) {
+        f3 x = mk(1.0f, 0.0f, 0.0f);  // smallest |component| of (0,1,0) with the reference's tie order
+        f3 u = x - from, w = x - wi;
+        float c1 = 2.0f / dot(u, u), c2 = 2.0f / dot(w, w), c3 = c1 * c2 * dot(u, w);
+        float ua[3] = {u.x, u.y, u.z}, wa[3] = {w.x, w.y, w.z};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) m[i][j] = -c1 * ua[i] * ua[j] - c2 * wa[i] * wa[j] + c3 * wa[i] * ua[j];
+            m[i][i] += 1.0f;
+        }
+    } else {
+        float h = 1.0f / (1.0f + e);
+        float hvx = h * vx.x, hvz = h * vx.z;
+        float hvxy = hvx * vx.y, hvxz = hvx * vx.z, hvyz = hvz * vx.y;
+        m[0][0] = e + hvx * vx.x;
+        m[0][1] = hvxy - vx.z;
+        m[0][2] = hvxz + vx.y;
+        m[1][0] = hvxy + vx.z;
+        m[1][1] = e + h * vx.y * vx.y;
+        m[1][2] = hvyz - vx.x;
+        m[2][0] = hvxz - vx.y;
+        m[2][1] = hvyz + vx.x;
+        m[2][2] = e + hvz * vx.z;
+    }
+    f3 s = mk(m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z, m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
+              m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z);
+    return normalize(normalize(s));
+}
+
+// ---------------------------------------------------------------------------------------------
+// shade: one thread per queued path whose ray has just been traced
+__global__ void __launch_bounds__(kBlock) shade_kernel(SceneDev S, RenderDev R, WaveDev Wv, int cur)
+{
+    const uint32_t count = Wv.counters[cur];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool live = i < count;
+        uint32_t slot = live ? Wv.q[cur][i] : 0u;
+        bool emit_shadow = false, cont = false;
+        float4 sh_o = {}, sh_d = {}, sh_c = {};
+        float4 n_o = {}, n_d = {}, n_thr = {};
+        uint4 n_misc = {}, n_extra = {};
+        if (live) {
+            float4 o4 = Wv.ray_o[slot], d4 = Wv.ray_d[slot], thr = Wv.thr[slot];
+            uint4 misc = Wv.misc[slot], extra = Wv.extra[slot];
+            float2 h = Wv.hit[slot];
+            int32_t tri = __float_as_int(h.y);
+            uint32_t level = extra.z;
+            f3 o = mk(o4), d = mk(d4);
+            float sampled = o4.w;
+            if (level == 0 && R.unwarp) extra.w = __float_as_uint(tri >= 0 ? h.x : 3.402823466e+38f);
+            // correct_next_event_distance (BidirectionalPathTracing.h:275-307)
+            bool medium_event = false;
+            float dist, pd = 1.0f;
+            bool alive = true;
+            if (!S.has_medium) {
+                alive = tri >= 0;
+                dist = h.x;
+            } else if (tri >= 0 && h.x < sampled) {
+                dist = h.x;
+                pd = expf(-dist * S.lambda);
+            } else {
+                dist = sampled;
+                medium_event = true;
+                pd = expf(-dist * S.lambda) * S.lambda;
+            }
+            if (alive) {
+                f3 f = mk(thr);
+                if (S.has_medium) {
+                    f = mk(f.x * expf(-dist * S.sigma_t.x), f.y * expf(-dist * S.sigma_t.y), f.z * expf(-dist * S.sigma_t.z));
+                    f = f * (1.0f / pd);
+                }
+                float time = thr.w + dist;
+                f3 pos = o + d * dist;
+                f3 n = mk(0.0f, 0.0f, 0.0f);
+                int32_t mat = -1;
+                if (!medium_event) {
+                    float4 nn = S.tri_normal[tri];
+                    n = mk(nn);
+                    mat = __float_as_int(nn.w);
+                }
+                uint64_t nst = u64(extra.x, extra.y);
+                uint64_t pid = u64(misc.x, misc.y);
+                uint64_t ninc = pcg_inc(2 * pid + 1), pinc = pcg_inc(2 * pid);
+                // ---- next-event estimation at this vertex (connect_paths :976-1045,
+                //      connect_vertex_with_light :650-692)
+                if (int(level) + 1 <= R.max_bounces && time <= R.total_time) {
+                    float u = pcg_float(nst, ninc);
+                    int li = min(int(u * float(S.n_lights)), S.n_lights - 1);
+                    const LightHost& L = S.lights[li];
+                    int type = int(L.I[3]);
+                    f3 lp, ldir;
+                    float ldist, pl2 = 1.0f;
+                    f3 irr;
+                    bool ok = true;
+                    if (type == kLightPoint) {
+                        lp = mk(L.pos[0], L.pos[1], L.pos[2]);
+                        f3 v = pos - lp;
+                        float d2 = dot(v, v);
+                        ldist = sqrtf(d2);
+                        ldir = v * (1.0f / ldist);
+                        float att = 1.0f / d2;
+                        irr = mk(L.I[0] * att, L.I[1] * att, L.I[2] * att);
+                    } else {
+                        float uu = pcg_float(nst, ninc), vv = pcg_float(nst, ninc);
+                        lp = mk(L.pos[0], L.pos[1], L.pos[2]) + mk(L.left[0], L.left[1], L.left[2]) * uu +
+                             mk(L.up[0], L.up[1], L.up[2]) * vv;
+                        f3 v = pos - lp;
+                        ldist = sqrtf(dot(v, v));
+                        ldir = v * (1.0f / ldist);
+                        float dl = dot(ldir, mk(L.normal[0], L.normal[1], L.normal[2]));
+                        ok = dl > 0.0f;
+                        float a = L.I[0] * dl / (ldist * ldist);
+                        irr = mk(a, a, a);
+                        pl2 = L.normal[3];
+                        ldir = normalize(ldir);
+                    }
+                    if (ok) {
+                        f3 fe;
+                        if (medium_event) {
+                            fe = mk(S.sigma_s.x, S.sigma_s.y, S.sigma_s.z) * kInv4Pi;
+                        } else {
+                            float4 kd = S.mats[mat];
+                            float c = fmaxf(-dot(ldir, n), 0.0f);
+                            fe = mk(kd.x, kd.y, kd.z) * (kInvPi * c);
+                        }
+                        f3 att = mk(1.0f, 1.0f, 1.0f);
+                        if (S.has_medium)
+                            att = mk(expf(-S.sigma_t.x * ldist), expf(-S.sigma_t.y * ldist), expf(-S.sigma_t.z * ldist));
+                        float scale = R.inv_inc / (pl2 / float(S.n_lights));
+                        f3 val = mul(mul(f, fe), mul(irr, att)) * scale;
+                        float ts = ldist + time;
+                        float cam_time = R.unwarp ? ts - __uint_as_float(extra.w) : ts;
+                        float sensor = (cam_time - R.offset) / R.exposure;
+                        bool finite = isfinite(val.x) && isfinite(val.y) && isfinite(val.z);
+                        bool nonzero = val.x != 0.0f || val.y != 0.0f || val.z != 0.0f;
+                        if (sensor >= 0.0f && sensor < float(R.T) && finite && nonzero) {
+                            uint32_t pix = uint32_t(pid / uint64_t(R.n_inc * R.spp2));  // (y*W + x)
+                            uint32_t rowl = pix / uint32_t(R.W) - uint32_t(R.row0);
+                            uint32_t x = pix % uint32_t(R.W);
+                            uint32_t bin = min(uint32_t(floorf(sensor)), uint32_t(R.T - 1));
+                            uint32_t fidx = (rowl * uint32_t(R.W) + x) * uint32_t(R.T) + bin;
+                            emit_shadow = true;
+                            sh_o = make_float4(lp.x, lp.y, lp.z, ldist);
+                            sh_d = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(fidx));
+                            sh_c = make_float4(val.x, val.y, val.z, 0.0f);
+                        }
+                    }
+                }
+                // ---- termination: later vertices cannot contribute past the bounce cap or the
+                //      film window; otherwise Russian roulette on the mean albedo (:447-481)
+                if (int(level) + 1 < R.max_bounces && time <= R.total_time) {
+                    uint64_t pst = u64(misc.z, misc.w);
+                    float albedo = medium_event ? S.medium_albedo : S.mats[mat].w;
+                    float eps = pcg_float(pst, pinc);
+                    if (!(eps > albedo)) {
+                        float pi = albedo;
+                        float nsampled = __int_as_float(0x7f800000);
+                        if (S.has_medium) {
+                            float e = pcg_float(pst, pinc);
+                            nsampled = -logf(e) / S.lambda;
+                        }
+                        f3 nd, fa;
+                        float pa;
+                        if (medium_event) {
+                            float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
+                            nd = sample_isotropic(d, e1, e2);
+                            fa = mk(S.sigma_s.x, S.sigma_s.y, S.sigma_s.z) * kInv4Pi;
+                            pa = kInv4Pi;
+                        } else {
+                            (void)pcg_float(pst, pinc);  // BSDF pick of a single-BSDF UberMaterial
+                            float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
+                            float ct;
+                            nd = sample_lambert(n, e1, e2, &ct);
+                            float4 kd = S.mats[mat];
+                            fa = mk(kd.x, kd.y, kd.z) * (kInvPi * fabsf(dot(n, nd)));
+                            pa = ct * kInvPi;
+                        }
+                        f3 nf = mul(f, fa) * (1.0f / (pi * pa));
+                        cont = true;
+                        n_o = make_float4(pos.x, pos.y, pos.z, nsampled);
+                        n_d = make_float4(nd.x, nd.y, nd.z, 0.0f);
+                        n_thr = make_float4(nf.x, nf.y, nf.z, time);
+                        n_misc = make_uint4(misc.x, misc.y, uint32_t(pst), uint32_t(pst >> 32));
+                        n_extra = make_uint4(uint32_t(nst), uint32_t(nst >> 32), level + 1, extra.w);
+                    }
+                }
+            }
+        }
+        uint32_t sh_pos = wave_append(Wv.counters + C_SHADOW, emit_shadow);
+        if (emit_shadow) {
+            Wv.sh_o[sh_pos] = sh_o;
+            Wv.sh_d[sh_pos] = sh_d;
+            Wv.sh_c[sh_pos] = sh_c;
+        }
+        uint32_t q_pos = wave_append(Wv.counters + (cur ^ 1), cont);
+        if (cont) {
+            Wv.ray_o[slot] = n_o;
+            Wv.ray_d[slot] = n_d;
+            Wv.thr[slot] = n_thr;
+            Wv.misc[slot] = n_misc;
+            Wv.extra[slot] = n_extra;
+            Wv.q[cur ^ 1][q_pos] = slot;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// shadow: occlusion test light -> vertex in (eps, dist - eps]; visible contributions go to the film
+__global__ void __launch_bounds__(kTraceBlock) shadow_kernel(SceneDev S, RenderDev R, WaveDev Wv)
+{
+    __shared__ int32_t stack[kBvhStack * kTraceBlock];
+    const uint32_t count = Wv.counters[C_SHADOW];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_SHADOW), (unsigned long long)count);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t visible = 0;
+    for (uint32_t i = gid; i < count; i += stride) {
+        float4 o = Wv.sh_o[i], d = Wv.sh_d[i];
+        RayPre r = ray_pre(mk(o), mk(d));
+        float tfar = o.w - kSigmaVisibility;
+        int32_t occ = trace<true>(S.nodes, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock);
+        if (occ < 0) {
+            float4 c = Wv.sh_c[i];
+            float* dst = R.film + size_t(__float_as_uint(d.w)) * 3;
+            unsafeAtomicAdd(dst + 0, c.x);
+            unsafeAtomicAdd(dst + 1, c.y);
+            unsafeAtomicAdd(dst + 2, c.z);
+            ++visible;
+        }
+    }
+    if (visible) atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_CONTRIB), (unsigned long long)visible);
+}
+
+// ---------------------------------------------------------------------------------------------
+// film merge: Neumaier-compensated accumulation of the pass buffer into (sum, comp); clears the pass
+__global__ void __launch_bounds__(kBlock) film_merge_kernel(float4* __restrict__ pass, float4* __restrict__ sum,
+                                                            float4* __restrict__ comp, size_t n4)
+{
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        float4 x = pass[i];
+        if (x.x == 0.0f && x.y == 0.0f && x.z == 0.0f && x.w == 0.0f) continue;
+        float4 s = sum[i], c = comp[i];
+        float xs[4] = {x.x, x.y, x.z, x.w}, ss[4] = {s.x, s.y, s.z, s.w}, cs[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float t = ss[k] + xs[k];
+            if (fabsf(ss[k]) >= fabsf(xs[k])) cs[k] += (ss[k] - t) + xs[k];
+            else cs[k] += (xs[k] - t) + ss[k];
+            ss[k] = t;
+        }
+        sum[i] = make_float4(ss[0], ss[1], ss[2], ss[3]);
+        comp[i] = make_float4(cs[0], cs[1], cs[2], cs[3]);
+        pass[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// batch closest hit for the public query entry point
+__global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, const tr_ray* __restrict__ rays,
+                                                                tr_hit* __restrict__ hits, const uint2* __restrict__ ids,
+                                                                uint32_t n)
+{
+    __shared__ int32_t stack[kBvhStack * kTraceBlock];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        tr_ray q = rays[i];
+        RayPre r = ray_pre(mk(q.org[0], q.org[1], q.org[2]), mk(q.dir[0], q.dir[1], q.dir[2]));
+        float tfar = q.tfar;
+        int32_t tri = trace<false>(S.nodes, S.tris, r, q.tnear, tfar, stack + threadIdx.x, kTraceBlock);
+        tr_hit h;
+        if (tri >= 0) {
+            uint2 gp = ids[tri];
+            h.t = tfar;
+            h.geom_id = gp.x;
+            h.prim_id = gp.y;
+            h.tri = uint32_t(tri);
+        } else {
+            h.t = -1.0f;
+            h.geom_id = 0xFFFFFFFFu;
+            h.prim_id = 0xFFFFFFFFu;
+            h.tri = 0xFFFFFFFFu;
+        }
+        hits[i] = h;
+    }
+}
+
+}  // namespace
+
+// =============================================================================================
+// host side
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(TR_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+int check_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(TR_ERR_DEVICE, "no HIP device available");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, dev));
+    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+        return fail(TR_ERR_DEVICE, std::string("device is ") + p.gcnArchName + ", the library is built for gfx950");
+    return TR_OK;
+}
+
+template <class T>
+int dev_alloc(T** p, size_t count, uint64_t* bytes = nullptr)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess) return fail(TR_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (bytes) *bytes += count * sizeof(T);
+    return TR_OK;
+}
+
+template <class T>
+void dev_free(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+struct WaveBuffers {
+    uint32_t cap = 0;
+    float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr;
+    uint4 *misc = nullptr, *extra = nullptr;
+    float2* hit = nullptr;
+    uint32_t *q0 = nullptr, *q1 = nullptr;
+    float4 *sh_o = nullptr, *sh_d = nullptr, *sh_c = nullptr;
+    uint32_t* counters = nullptr;
+
+    void release()
+    {
+        dev_free(ray_o); dev_free(ray_d); dev_free(thr); dev_free(misc); dev_free(extra); dev_free(hit);
+        dev_free(q0); dev_free(q1); dev_free(sh_o); dev_free(sh_d); dev_free(sh_c); dev_free(counters);
+        cap = 0;
+    }
+    int ensure(uint32_t n)
+    {
+        if (n <= cap) return TR_OK;
+        release();
+        int rc;
+        if ((rc = dev_alloc(&ray_o, n)) || (rc = dev_alloc(&ray_d, n)) || (rc = dev_alloc(&thr, n)) ||
+            (rc = dev_alloc(&misc, n)) || (rc = dev_alloc(&extra, n)) || (rc = dev_alloc(&hit, n)) ||
+            (rc = dev_alloc(&q0, n)) || (rc = dev_alloc(&q1, n)) || (rc = dev_alloc(&sh_o, n)) ||
+            (rc = dev_alloc(&sh_d, n)) || (rc = dev_alloc(&sh_c, n)) || (rc = dev_alloc(&counters, C_WORDS))) {
+            release();
+            return rc;
+        }
+        cap = n;
+        return TR_OK;
+    }
+    WaveDev view() const
+    {
+        WaveDev w;
+        w.ray_o = ray_o; w.ray_d = ray_d; w.thr = thr; w.misc = misc; w.extra = extra; w.hit = hit;
+        w.q[0] = q0; w.q[1] = q1; w.sh_o = sh_o; w.sh_d = sh_d; w.sh_c = sh_c; w.counters = counters;
+        return w;
+    }
+};
+
+}  // namespace
+
+struct tr_scene_s {
+    std::vector<float> tris;        // n*9
+    std::vector<int32_t> tri_mat;
+    std::vector<uint32_t> tri_geom, tri_prim;
+    std::vector<float> mats;        // n_mat*4
+    std::vector<LightHost> lights;
+    uint32_t n_geoms = 0;
+    bool has_medium = false;
+    float sa[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
+    bool committed = false;
+    hipStream_t stream = nullptr;
+    float4 *d_nodes = nullptr, *d_tris = nullptr, *d_tri_normal = nullptr, *d_mats = nullptr;
+    LightHost* d_lights = nullptr;
+    uint2* d_ids = nullptr;
+    uint32_t n_nodes = 0;
+    int depth = 0;
+    double build_ms = 0.0;
+    uint64_t dev_bytes = 0;
+    WaveBuffers wave;
+    hipEvent_t ev[2 * 256] = {};
+    bool ev_made = false;
+
+    void release_device()
+    {
+        dev_free(d_nodes); dev_free(d_tris); dev_free(d_tri_normal); dev_free(d_mats); dev_free(d_lights);
+        dev_free(d_ids);
+        dev_bytes = 0;
+    }
+    SceneDev view() const
+    {
+        SceneDev s;
+        s.nodes = d_nodes;
+        s.tris = d_tris;
+        s.tri_normal = d_tri_normal;
+        s.mats = d_mats;
+        s.lights = d_lights;
+        s.n_lights = int32_t(lights.size());
+        s.has_medium = has_medium ? 1 : 0;
+        s.sigma_s = make_float3(ss[0], ss[1], ss[2]);
+        s.sigma_t = make_float3(sa[0] + ss[0], sa[1] + ss[1], sa[2] + ss[2]);
+        s.lambda = (s.sigma_t.x + s.sigma_t.y + s.sigma_t.z) / 3.0f;
+        s.medium_albedo = has_medium ? (ss[0] / s.sigma_t.x + ss[1] / s.sigma_t.y + ss[2] / s.sigma_t.z) / 3.0f : 0.f;
+        return s;
+    }
+};
+
+struct tr_film_s {
+    tr_film_desc desc{};
+    int rows = 0;
+    size_t n_elems = 0;  // rows*W*T*3
+    float *d_pass = nullptr, *d_sum = nullptr, *d_comp = nullptr;
+    std::vector<double> weight;  // rows*W
+    hipStream_t stream = nullptr;
+};
+
+extern "C" {
+
+const char* tr_last_error(void) { return g_err.c_str(); }
+
+int tr_device_info(char* name, int name_len, int* compute_units, uint64_t* global_mem)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(TR_ERR_DEVICE, "no HIP device available");
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, 0));
+    if (name && name_len > 0) {
+        std::snprintf(name, size_t(name_len), "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (compute_units) *compute_units = p.multiProcessorCount;
+    if (global_mem) *global_mem = p.totalGlobalMem;
+    return TR_OK;
+}
+
+int tr_scene_create(tr_scene* out)
+{
+    if (!out) return fail(TR_ERR_INVALID, "tr_scene_create: null output");
+    *out = nullptr;
+    int rc = check_device();
+    if (rc) return rc;
+    tr_scene s = new (std::nothrow) tr_scene_s();
+    if (!s) return fail(TR_ERR_NOMEM, "tr_scene_create: host allocation");
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return fail(TR_ERR_DEVICE, "tr_scene_create: hipStreamCreate failed");
+    }
+    *out = s;
+    return TR_OK;
+}
+
+int tr_scene_destroy(tr_scene s)
+{
+    if (!s) return TR_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->release_device();
+    s->wave.release();
+    if (s->ev_made)
+        for (auto& e : s->ev) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return TR_OK;
+}
+
+int tr_scene_add_mesh(tr_scene s, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                      const tr_material* m, uint32_t* geom_id)
+{
+    if (!s || (nt > 0 && (!xyz || !idx)) || !m) return fail(TR_ERR_INVALID, "tr_scene_add_mesh: null argument");
+    for (uint32_t i = 0; i < 3 * nt; ++i)
+        if (idx[i] >= nv) return fail(TR_ERR_INVALID, "tr_scene_add_mesh: vertex index out of range");
+    for (int k = 0; k < 3; ++k)
+        if (!(m->kd[k] >= 0.f && m->kd[k] <= 1.f)) return fail(TR_ERR_INVALID, "tr_scene_add_mesh: Kd outside [0,1]");
+    if (s->tris.size() / 9 + nt > 0x7FFFFFFFu) return fail(TR_ERR_INVALID, "tr_scene_add_mesh: too many triangles");
+    int32_t mat = int32_t(s->mats.size() / 4);
+    // mean albedo used by the roulette: mean of 1 - (1 - Kd) (BidirectionalPathTracing.h:632-638)
+    float alb = ((1.f - (1.f - m->kd[0])) + (1.f - (1.f - m->kd[1])) + (1.f - (1.f - m->kd[2]))) / 3.0f;
+    s->mats.insert(s->mats.end(), {m->kd[0], m->kd[1], m->kd[2], alb});
+    uint32_t g = s->n_geoms++;
+    for (uint32_t t = 0; t < nt; ++t) {
+        for (int k = 0; k < 3; ++k) {
+            const float* p = xyz + 3 * size_t(idx[3 * t + k]);
+            s->tris.insert(s->tris.end(), {p[0], p[1], p[2]});
+        }
+        s->tri_mat.push_back(mat);
+        s->tri_geom.push_back(g);
+        s->tri_prim.push_back(t);
+    }
+    if (geom_id) *geom_id = g;
+    s->committed = false;
+    return TR_OK;
+}
+
+int tr_scene_add_point_light(tr_scene s, const float position[3], const float intensity[3])
+{
+    if (!s || !position || !intensity) return fail(TR_ERR_INVALID, "tr_scene_add_point_light: null argument");
+    LightHost L{};
+    std::memcpy(L.pos, position, 12);
+    std::memcpy(L.I, intensity, 12);
+    L.I[3] = float(kLightPoint);
+    s->lights.push_back(L);
+    s->committed = false;
+    return TR_OK;
+}
+
+int tr_scene_add_rect_light(tr_scene s, const float corner[3], const float target[3], const float size[2],
+                            float intensity)
+{
+    if (!s || !corner || !target || !size) return fail(TR_ERR_INVALID, "tr_scene_add_rect_light: null argument");
+    if (!(size[0] > 0.f && size[1] > 0.f)) return fail(TR_ERR_INVALID, "tr_scene_add_rect_light: non-positive size");
+    // RectangularAreaLightSource::setup (RectangularAreaLightSource.h:66-80)
+    double n[3] = {double(target[0]) - corner[0], double(target[1]) - corner[1], double(target[2]) - corner[2]};
+    float nf[3] = {float(target[0] - corner[0]), float(target[1] - corner[1]), float(target[2] - corner[2])};
+    float len = std::sqrt(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]);
+    if (!(len > 0.f)) return fail(TR_ERR_INVALID, "tr_scene_add_rect_light: target equals corner");
+    (void)n;
+    float inv = float(1. / double(len));
+    float N[3] = {nf[0] * inv, nf[1] * inv, nf[2] * inv};
+    float up[3] = {0.f, 1.f, 0.f};
+    if (!(std::fabs(double(N[1])) < 0.9999)) {
+        up[1] = 0.f;
+        up[2] = 1.f;
+    }
+    auto crs = [](const float* a, const float* b, float* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    float left[3], up2[3];
+    crs(N, up, left);
+    crs(N, left, up2);
+    LightHost L{};
+    std::memcpy(L.pos, corner, 12);
+    for (int k = 0; k < 3; ++k) {
+        L.left[k] = left[k] * size[0];
+        L.up[k] = up2[k] * size[1];
+        L.normal[k] = N[k];
+        L.I[k] = intensity;
+    }
+    L.normal[3] = float(1. / double(size[0] * size[1]));
+    L.I[3] = float(kLightRect);
+    s->lights.push_back(L);
+    s->committed = false;
+    return TR_OK;
+}
+
+int tr_scene_set_medium(tr_scene s, const float sigma_a[3], const float sigma_s[3])
+{
+    if (!s) return fail(TR_ERR_INVALID, "tr_scene_set_medium: null scene");
+    if (!sigma_a || !sigma_s) {
+        s->has_medium = false;
+        return TR_OK;
+    }
+    for (int k = 0; k < 3; ++k)
+        if (!(sigma_a[k] >= 0.f && sigma_s[k] >= 0.f && sigma_a[k] + sigma_s[k] > 0.f))
+            return fail(TR_ERR_INVALID, "tr_scene_set_medium: coefficients must be >= 0 with positive extinction");
+    std::memcpy(s->sa, sigma_a, 12);
+    std::memcpy(s->ss, sigma_s, 12);
+    s->has_medium = true;
+    s->committed = false;
+    return TR_OK;
+}
+
+int tr_scene_commit(tr_scene s)
+{
+    if (!s) return fail(TR_ERR_INVALID, "tr_scene_commit: null scene");
+    uint32_t n = uint32_t(s->tri_mat.size());
+    BvhBuildResult b = build_bvh(s->tris.data(), s->tri_mat.data(), n);
+    if (b.depth >= kBvhStack) return fail(TR_ERR_INVALID, "tr_scene_commit: BVH deeper than the traversal stack");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->release_device();
+    std::vector<float4> normals(n > 0 ? n : 1);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* t = &s->tris[9 * size_t(i)];
+        // ng = normalize(cross(p2 - p0, p1 - p0)) (World.h:449)
+        float e2[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]}, e1[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]};
+        float c[3] = {e2[1] * e1[2] - e2[2] * e1[1], e2[2] * e1[0] - e2[0] * e1[2], e2[0] * e1[1] - e2[1] * e1[0]};
+        float l = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        float inv = l > 0.f ? float(1. / double(l)) : 0.f;
+        int32_t mi = s->tri_mat[i];
+        float w;
+        std::memcpy(&w, &mi, 4);
+        normals[i] = make_float4(c[0] * inv, c[1] * inv, c[2] * inv, w);
+    }
+    std::vector<uint2> ids(n > 0 ? n : 1);
+    for (uint32_t i = 0; i < n; ++i) ids[i] = make_uint2(s->tri_geom[i], s->tri_prim[i]);
+    size_t nn = b.nodes.size(), nt = b.tris.size(), nm = std::max<size_t>(1, s->mats.size() / 4),
+           nl = std::max<size_t>(1, s->lights.size());
+    int rc;
+    if ((rc = dev_alloc(&s->d_nodes, 4 * nn, &s->dev_bytes)) || (rc = dev_alloc(&s->d_tris, 3 * std::max<size_t>(1, nt), &s->dev_bytes)) ||
+        (rc = dev_alloc(&s->d_tri_normal, normals.size(), &s->dev_bytes)) || (rc = dev_alloc(&s->d_mats, nm, &s->dev_bytes)) ||
+        (rc = dev_alloc(&s->d_lights, nl, &s->dev_bytes)) || (rc = dev_alloc(&s->d_ids, ids.size(), &s->dev_bytes))) {
+        s->release_device();
+        return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(s->d_nodes, b.nodes.data(), nn * sizeof(BvhNodeHost), hipMemcpyHostToDevice, s->stream));
+    if (nt) HIP_TRY(hipMemcpyAsync(s->d_tris, b.tris.data(), nt * sizeof(TriHost), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_tri_normal, normals.data(), normals.size() * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    if (!s->mats.empty())
+        HIP_TRY(hipMemcpyAsync(s->d_mats, s->mats.data(), s->mats.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    if (!s->lights.empty())
+        HIP_TRY(hipMemcpyAsync(s->d_lights, s->lights.data(), s->lights.size() * sizeof(LightHost), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_ids, ids.data(), ids.size() * sizeof(uint2), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->n_nodes = uint32_t(nn);
+    s->depth = b.depth;
+    s->build_ms = b.build_ms;
+    s->committed = true;
+    return TR_OK;
+}
+
+int tr_scene_info_get(tr_scene s, tr_scene_info* info)
+{
+    if (!s || !info) return fail(TR_ERR_INVALID, "tr_scene_info_get: null argument");
+    info->n_tris = uint32_t(s->tri_mat.size());
+    info->n_bvh_nodes = s->n_nodes;
+    info->bvh_depth = uint32_t(s->depth);
+    info->n_lights = uint32_t(s->lights.size());
+    info->device_bytes = s->dev_bytes;
+    info->build_ms = s->build_ms;
+    return TR_OK;
+}
+
+static uint32_t trace_grid(uint32_t n)
+{
+    uint32_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
+    return std::max(1u, std::min(blocks, 256u * 40u));
+}
+
+int tr_intersect_batch_device(tr_scene s, const tr_ray* d_rays, uint32_t n, tr_hit* d_hits, void* stream)
+{
+    if (!s) return fail(TR_ERR_INVALID, "tr_intersect_batch_device: null scene");
+    if (!s->committed) return fail(TR_ERR_STATE, "tr_intersect_batch_device: scene not committed");
+    if (n == 0) return TR_OK;
+    if (!d_rays || !d_hits) return fail(TR_ERR_INVALID, "tr_intersect_batch_device: null buffer");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+    intersect_kernel<<<trace_grid(n), kTraceBlock, 0, st>>>(s->view(), d_rays, d_hits, s->d_ids, n);
+    HIP_TRY(hipGetLastError());
+    return TR_OK;
+}
+
+int tr_intersect_batch(tr_scene s, const tr_ray* rays, uint32_t n, tr_hit* hits)
+{
+    if (!s) return fail(TR_ERR_INVALID, "tr_intersect_batch: null scene");
+    if (!s->committed) return fail(TR_ERR_STATE, "tr_intersect_batch: scene not committed");
+    if (n == 0) return TR_OK;
+    if (!rays || !hits) return fail(TR_ERR_INVALID, "tr_intersect_batch: null buffer");
+    tr_ray* dr = nullptr;
+    tr_hit* dh = nullptr;
+    int rc;
+    if ((rc = dev_alloc(&dr, n)) || (rc = dev_alloc(&dh, n))) {
+        dev_free(dr);
+        return rc;
+    }
+    int out = TR_OK;
+    if (hipMemcpyAsync(dr, rays, size_t(n) * sizeof(tr_ray), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        out = fail(TR_ERR_DEVICE, "tr_intersect_batch: upload failed");
+    if (!out) out = tr_intersect_batch_device(s, dr, n, dh, s->stream);
+    if (!out && hipMemcpyAsync(hits, dh, size_t(n) * sizeof(tr_hit), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+        out = fail(TR_ERR_DEVICE, "tr_intersect_batch: download failed");
+    if (!out && hipStreamSynchronize(s->stream) != hipSuccess) out = fail(TR_ERR_DEVICE, "tr_intersect_batch: kernel failed");
+    dev_free(dr);
+    dev_free(dh);
+    return out;
+}
+
+int tr_film_create(tr_film* out, const tr_film_desc* d)
+{
+    if (!out || !d) return fail(TR_ERR_INVALID, "tr_film_create: null argument");
+    *out = nullptr;
+    if (d->width <= 0 || d->height <= 0 || d->time_bins <= 0 || !(d->exposure > 0.f) ||
+        d->scanline >= d->height || d->scanline < -1)
+        return fail(TR_ERR_INVALID, "tr_film_create: bad film geometry");
+    int rc = check_device();
+    if (rc) return rc;
+    int rows = d->scanline >= 0 ? 1 : d->height;
+    size_t n = size_t(rows) * size_t(d->width) * size_t(d->time_bins) * 3;
+    if (n / 3 >= 0xFFFFFFFFull) return fail(TR_ERR_INVALID, "tr_film_create: film larger than 2^32 bins");
+    tr_film f = new (std::nothrow) tr_film_s();
+    if (!f) return fail(TR_ERR_NOMEM, "tr_film_create: host allocation");
+    f->desc = *d;
+    f->rows = rows;
+    f->n_elems = n;
+    size_t n_alloc = (n + 3) / 4 * 4;  // merge kernel works on float4
+    if ((rc = dev_alloc(&f->d_pass, n_alloc)) || (rc = dev_alloc(&f->d_sum, n_alloc)) || (rc = dev_alloc(&f->d_comp, n_alloc))) {
+        dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp);
+        delete f;
+        return rc;
+    }
+    if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess) {
+        dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp);
+        delete f;
+        return fail(TR_ERR_DEVICE, "tr_film_create: hipStreamCreate failed");
+    }
+    f->weight.assign(size_t(rows) * size_t(d->width), 0.0);
+    *out = f;
+    rc = tr_film_clear(f);
+    if (rc) {
+        tr_film_destroy(f);
+        *out = nullptr;
+    }
+    return rc;
+}
+
+int tr_film_destroy(tr_film f)
+{
+    if (!f) return TR_OK;
+    if (f->stream) (void)hipStreamSynchronize(f->stream);
+    dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp);
+    if (f->stream) (void)hipStreamDestroy(f->stream);
+    delete f;
+    return TR_OK;
+}
+
+int tr_film_clear(tr_film f)
+{
+    if (!f) return fail(TR_ERR_INVALID, "tr_film_clear: null film");
+    size_t bytes = (f->n_elems + 3) / 4 * 4 * sizeof(float);
+    HIP_TRY(hipMemsetAsync(f->d_pass, 0, bytes, f->stream));
+    HIP_TRY(hipMemsetAsync(f->d_sum, 0, bytes, f->stream));
+    HIP_TRY(hipMemsetAsync(f->d_comp, 0, bytes, f->stream));
+    HIP_TRY(hipStreamSynchronize(f->stream));
+    std::fill(f->weight.begin(), f->weight.end(), 0.0);
+    return TR_OK;
+}
+
+int tr_film_read_sums(tr_film f, double* sums, double* weights)
+{
+    if (!f) return fail(TR_ERR_INVALID, "tr_film_read_sums: null film");
+    if (sums) {
+        std::vector<float> s(f->n_elems), c(f->n_elems);
+        HIP_TRY(hipMemcpyAsync(s.data(), f->d_sum, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+        HIP_TRY(hipMemcpyAsync(c.data(), f->d_comp, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+        HIP_TRY(hipStreamSynchronize(f->stream));
+        for (size_t i = 0; i < f->n_elems; ++i) sums[i] = double(s[i]) + double(c[i]);
+    }
+    if (weights) std::copy(f->weight.begin(), f->weight.end(), weights);
+    return TR_OK;
+}
+
+int tr_film_read(tr_film f, float* out, float* steady)
+{
+    if (!f) return fail(TR_ERR_INVALID, "tr_film_read: null film");
+    if (!out && !steady) return TR_OK;
+    std::vector<float> s(f->n_elems), c(f->n_elems);
+    HIP_TRY(hipMemcpyAsync(s.data(), f->d_sum, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+    HIP_TRY(hipMemcpyAsync(c.data(), f->d_comp, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+    HIP_TRY(hipStreamSynchronize(f->stream));
+    const size_t W = size_t(f->desc.width), T = size_t(f->desc.time_bins);
+    for (size_t p = 0; p < size_t(f->rows) * W; ++p) {
+        double w = f->weight[p];
+        double acc[3] = {0, 0, 0};
+        for (size_t t = 0; t < T; ++t)
+            for (int k = 0; k < 3; ++k) {
+                size_t i = (p * T + t) * 3 + size_t(k);
+                double v = (double(s[i]) + double(c[i])) / w;
+                if (!std::isfinite(v)) v = 0.0;  // 0/0 -> 0 (ImageRow::weight, Image.h:153-162)
+                if (out) out[i] = float(v);
+                acc[k] += v;
+            }
+        if (steady)
+            for (int k = 0; k < 3; ++k) steady[p * 3 + size_t(k)] = float(acc[k]);
+    }
+    return TR_OK;
+}
+
+int tr_film_device_buffers(tr_film f, float** d_sum, float** d_comp)
+{
+    if (!f) return fail(TR_ERR_INVALID, "tr_film_device_buffers: null film");
+    if (d_sum) *d_sum = f->d_sum;
+    if (d_comp) *d_comp = f->d_comp;
+    return TR_OK;
+}
+
+int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* d, tr_render_stats* stats)
+{
+    auto t_start = std::chrono::steady_clock::now();
+    if (!s || !cam || !f || !d) return fail(TR_ERR_INVALID, "tr_render_transient: null argument");
+    if (!s->committed) return fail(TR_ERR_STATE, "tr_render_transient: scene not committed");
+    if (s->lights.empty()) return fail(TR_ERR_INVALID, "tr_render_transient: scene has no light");
+    if (d->sqrt_spp <= 0 || d->n_incoming <= 0 || d->max_bounces < 0)
+        return fail(TR_ERR_INVALID, "tr_render_transient: bad sampling parameters");
+    const tr_film_desc& fd = f->desc;
+    const uint64_t spp2 = uint64_t(d->sqrt_spp) * uint64_t(d->sqrt_spp);
+    const uint64_t total = uint64_t(f->rows) * uint64_t(fd.width) * spp2 * uint64_t(d->n_incoming);
+    if ((uint64_t(fd.height) * fd.width * spp2 * uint64_t(d->n_incoming)) >= (1ull << 61))
+        return fail(TR_ERR_INVALID, "tr_render_transient: too many paths for the RNG stream ids");
+    uint32_t wave = d->wave_paths ? d->wave_paths : kDefaultWave;
+    wave = uint32_t(std::min<uint64_t>(wave, std::max<uint64_t>(total, 1)));
+    int rc = s->wave.ensure(wave);
+    if (rc) return rc;
+
+    // camera set-up (PinholeCamera.h:56-80; bunnykiller.cpp:1156-1160)
+    auto nrm = [](float* v) {
+        float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        float inv = float(1. / double(l));
+        for (int k = 0; k < 3; ++k) v[k] *= inv;
+    };
+    auto crs = [](const float* a, const float* b, float* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    float up[3] = {cam->up[0], cam->up[1], cam->up[2]};
+    nrm(up);
+    float nv[3] = {cam->focus[0] - cam->position[0], cam->focus[1] - cam->position[1], cam->focus[2] - cam->position[2]};
+    nrm(nv);
+    float uv[3], vv[3];
+    crs(nv, up, uv);
+    nrm(uv);
+    crs(uv, nv, vv);
+    nrm(vv);
+    float fov = float(M_PI / 180. * double(cam->fov_deg) / 2);
+    RenderDev R{};
+    R.film = f->d_pass;
+    R.W = fd.width;
+    R.H = fd.height;
+    R.T = fd.time_bins;
+    R.row0 = fd.scanline >= 0 ? fd.scanline : 0;
+    R.spp = d->sqrt_spp;
+    R.spp2 = int32_t(spp2);
+    R.n_inc = d->n_incoming;
+    R.exposure = fd.exposure;
+    R.offset = fd.offset;
+    R.total_time = float(fd.time_bins) * fd.exposure + fd.offset;
+    R.inv_inc = float(1. / double(float(d->n_incoming)));
+    R.unwarp = fd.camera_unwarp ? 1 : 0;
+    R.max_bounces = d->max_bounces;
+    R.cpos = make_float3(cam->position[0], cam->position[1], cam->position[2]);
+    R.nv = make_float3(nv[0], nv[1], nv[2]);
+    R.uv = make_float3(uv[0], uv[1], uv[2]);
+    R.vv = make_float3(vv[0], vv[1], vv[2]);
+    R.fd = float(1. / std::tan(double(fov)));
+    R.xw2 = float(double(fd.width) / 2.0);
+    R.yw2 = float(double(fd.height) / 2.0);
+    R.ratio = float(fd.width) / float(fd.height);
+    R.inv_spp = float(1. / double(float(d->sqrt_spp)));
+    R.seed = d->seed;
+
+    const bool prof = (d->flags & TR_RENDER_PROFILE) != 0;
+    if (prof && !s->ev_made) {
+        for (auto& e : s->ev) HIP_TRY(hipEventCreate(&e));
+        s->ev_made = true;
+    }
+    std::vector<std::pair<int, int>> ev_kind;  // (kind, event pair index)
+    double kernel_ms[TR_K_COUNT] = {};
+    uint32_t launches[TR_K_COUNT] = {};
+    auto flush_events = [&]() -> int {
+        if (ev_kind.empty()) return TR_OK;
+        HIP_TRY(hipEventSynchronize(s->ev[2 * ev_kind.back().second + 1]));
+        for (auto& kv : ev_kind) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, s->ev[2 * kv.second], s->ev[2 * kv.second + 1]));
+            kernel_ms[kv.first] += ms;
+        }
+        ev_kind.clear();
+        return TR_OK;
+    };
+    auto begin = [&](int kind) -> int {
+        launches[kind]++;
+        if (!prof) return TR_OK;
+        if (ev_kind.size() == 256) {
+            int r2 = flush_events();
+            if (r2) return r2;
+        }
+        int slot = int(ev_kind.size());
+        ev_kind.push_back({kind, slot});
+        HIP_TRY(hipEventRecord(s->ev[2 * slot], s->stream));
+        return TR_OK;
+    };
+    auto end = [&]() -> int {
+        HIP_TRY(hipGetLastError());
+        if (!prof) return TR_OK;
+        HIP_TRY(hipEventRecord(s->ev[2 * ev_kind.back().second + 1], s->stream));
+        return TR_OK;
+    };
+
+    SceneDev S = s->view();
+    WaveDev Wv = s->wave.view();
+    HIP_TRY(hipMemsetAsync(s->wave.counters, 0, C_WORDS * sizeof(uint32_t), s->stream));
+    // the pass buffer is written on the scene stream; order it after any pending film work
+    HIP_TRY(hipStreamSynchronize(f->stream));
+
+    uint32_t n_waves = 0;
+    int bounces = d->max_bounces;
+    for (uint64_t base = 0; base < total; base += wave) {
+        uint32_t n = uint32_t(std::min<uint64_t>(wave, total - base));
+        ++n_waves;
+        if ((rc = begin(TR_K_CAMERA))) return rc;
+        camera_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(S, R, Wv, base, n);
+        if ((rc = end())) return rc;
+        uint32_t tg = trace_grid(n);
+        uint32_t sg = std::max(1u, std::min((n + kBlock - 1) / kBlock, 256u * 16u));
+        int cur = 0;
+        for (int b = 0; b < bounces; ++b) {
+            if ((rc = begin(TR_K_EXTEND))) return rc;
+            extend_kernel<<<tg, kTraceBlock, 0, s->stream>>>(S, Wv, cur);
+            if ((rc = end())) return rc;
+            if ((rc = begin(TR_K_SHADE))) return rc;
+            shade_kernel<<<sg, kBlock, 0, s->stream>>>(S, R, Wv, cur);
+            if ((rc = end())) return rc;
+            if ((rc = begin(TR_K_SHADOW))) return rc;
+            shadow_kernel<<<tg, kTraceBlock, 0, s->stream>>>(S, R, Wv);
+            if ((rc = end())) return rc;
+            cur ^= 1;
+        }
+    }
+    {
+        size_t n4 = (f->n_elems + 3) / 4;
+        uint32_t mg = uint32_t(std::max<size_t>(1, std::min<size_t>((n4 + kBlock - 1) / kBlock, 256u * 64u)));
+        if ((rc = begin(TR_K_FILM_MERGE))) return rc;
+        film_merge_kernel<<<mg, kBlock, 0, s->stream>>>(reinterpret_cast<float4*>(f->d_pass),
+                                                         reinterpret_cast<float4*>(f->d_sum),
+                                                         reinterpret_cast<float4*>(f->d_comp), n4);
+        if ((rc = end())) return rc;
+    }
+    uint32_t counters[C_WORDS];
+    HIP_TRY(hipMemcpyAsync(counters, s->wave.counters, sizeof counters, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if ((rc = flush_events())) return rc;
+    for (double& w : f->weight) w += double(spp2);
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->paths = total;
+        auto rd64 = [&](int w) { return uint64_t(counters[w]) | (uint64_t(counters[w + 1]) << 32); };
+        stats->extension_rays = rd64(C_STAT_EXT);
+        stats->shadow_rays = rd64(C_STAT_SHADOW);
+        stats->contributions = rd64(C_STAT_CONTRIB);
+        stats->waves = n_waves;
+        stats->bounces_max = uint32_t(bounces);
+        for (int k = 0; k < TR_K_COUNT; ++k) {
+            stats->kernel_ms[k] = kernel_ms[k];
+            stats->kernel_launches[k] = launches[k];
+        }
+        stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    }
+    return TR_OK;
+}
+
+}  // extern "C"

@@ -21,6 +21,7 @@
 #include <float.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -388,14 +389,21 @@ static inline int slab(const bvh_node* n, v3 O, v3 inv, float tnear, float tfar)
     float t0 = tnear, t1 = tfar;
     float o[3] = {O.x, O.y, O.z}, id[3] = {inv.x, inv.y, inv.z};
     for (int k = 0; k < 3; ++k) {
+        /* inv is finite (zero direction components are clamped), so no 0 * inf */
         float a = (n->lo[k] - o[k]) * id[k], b = (n->hi[k] - o[k]) * id[k];
         float lo = a < b ? a : b, hi = a < b ? b : a;
-        if (lo != lo) lo = -FLT_MAX; /* 0 * inf */
-        if (hi != hi) hi = FLT_MAX;
         if (lo > t0) t0 = lo;
         if (hi < t1) t1 = hi;
     }
-    return t0 <= t1 * 1.00000024f;
+    /* three roundings per bound: widen the far bound by 1 + 2*gamma(3) (Pharr et al., PBRT 3.9.2) */
+    return t0 <= t1 * 1.00000036f;
+}
+
+/* reciprocal direction for slab tests, zero components replaced by +-1e-20 */
+static inline float safe_rcp(float d)
+{
+    if (fabsf(d) < 1e-20f) d = copysignf(1e-20f, d);
+    return 1.0f / d;
 }
 
 /* closest hit, method = ORC_ISECT_*; returns triangle index or -1; *t is updated */
@@ -414,7 +422,7 @@ static int trace(const world_t* w, v3 O, v3 D, float tnear, float* t, int method
         return hit;
     }
     if (w->s->n_tris == 0) return -1;
-    v3 inv = V(1.0f / D.x, 1.0f / D.y, 1.0f / D.z);
+    v3 inv = V(safe_rcp(D.x), safe_rcp(D.y), safe_rcp(D.z));
     int32_t stack[128]; int sp = 0;
     stack[sp++] = 0;
     while (sp > 0) {
@@ -773,6 +781,8 @@ int orc_render(const orc_scene* s, const orc_render_desc* d, double* film_sum, d
     R.mode = d->rng_mode == ORC_RNG_PER_PATH ? ORC_RNG_PER_PATH : ORC_RNG_REFERENCE;
     R.global = &global;
 
+    struct timespec ts0, ts1;
+    clock_gettime(CLOCK_MONOTONIC, &ts0);
     for (int y = row0; y < row1; ++y) {
         for (int x = 0; x < W; ++x) {
             float limit_x = nextafterf((float)(x + 1), 0.0f), limit_y = nextafterf((float)(y + 1), 0.0f);
@@ -836,6 +846,8 @@ int orc_render(const orc_scene* s, const orc_render_desc* d, double* film_sum, d
             }
         }
     }
+    clock_gettime(CLOCK_MONOTONIC, &ts1);
+    st.render_seconds = (double)(ts1.tv_sec - ts0.tv_sec) + 1e-9 * (double)(ts1.tv_nsec - ts0.tv_nsec);
     free(verts); free(smp);
     free(c.w.tris); free(c.w.nodes); free(c.w.prim); free(c.rects);
     if (stats) *stats = st;

@@ -64,6 +64,7 @@ typedef struct {
 
 typedef struct {
     uint64_t camera_samples, paths, extension_rays, shadow_rays, contributions;
+    double render_seconds; /* the sample loop only (scene set-up and BVH build excluded) */
 } orc_stats;
 
 /* PCG32 (RandomNumbers.h:119-174) */

@@ -47,10 +47,12 @@ class _Render(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("camera_samples", ctypes.c_uint64), ("paths", ctypes.c_uint64),
                 ("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
-                ("contributions", ctypes.c_uint64)]
+                ("contributions", ctypes.c_uint64), ("render_seconds", ctypes.c_double)]
 
     def as_dict(self):
-        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+        d = {k: int(getattr(self, k)) for k, _ in self._fields_[:-1]}
+        d["render_seconds"] = float(self.render_seconds)
+        return d
 
 
 class _PCG(ctypes.Structure):

@@ -260,8 +260,22 @@ static int run_intersect(const SceneDesc& s, const char* rays_path, const char* 
     return 0;
 }
 
+// The reference's global RNG after Random::StdRNG.seed(state[, seq]): n integers and n floats.
+static int run_rng(int argc, char** argv)
+{
+    unsigned long long state = strtoull(argv[2], nullptr, 0);
+    int n = atoi(argv[3]);
+    if (argc >= 5) Random::StdRNG.seed(state, strtoull(argv[4], nullptr, 0));
+    else Random::StdRNG.seed(state);
+    Random::RNG copy(Random::StdRNG);
+    for (int i = 0; i < n; ++i) printf("%u%c", Random::StdRNG.next_int(), i + 1 < n ? ' ' : '\n');
+    for (int i = 0; i < n; ++i) printf("%.9g%c", copy.next_float(), i + 1 < n ? ' ' : '\n');
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
+    if (argc >= 4 && !strcmp(argv[1], "rng")) return run_rng(argc, argv);
     if (argc >= 3 && !strcmp(argv[1], "render")) return run_render(parse_scene(argv[2]), false);
     if (argc >= 3 && !strcmp(argv[1], "render_cs")) return run_render(parse_scene(argv[2]), true);
     if (argc >= 5 && !strcmp(argv[1], "intersect"))

@@ -1,0 +1,370 @@
+"""CPU tests that pin the oracle (oracle/oracle.c) against the reference's own outputs
+(tests/golden/, written by the reference's classes through oracle/ref_harness.cpp), published
+known answers and closed forms. No GPU needed."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from jarabotransientrendering_amd import scenes
+from tests.conftest import GOLDEN
+
+# pcg32-demo (O'Neill, pcg-c-basic) published output for pcg32_srandom(42, 54)
+PCG32_DEMO_42_54 = [0xA15C02B7, 0x7B47F409, 0xBA1D3330, 0x83D2F293, 0xBFA4784B, 0xCBED606E]
+
+
+def test_pcg32_published_known_answer(oracle):
+    r = oracle.PCG32(42, 54)
+    assert [r.next_int() for _ in range(6)] == PCG32_DEMO_42_54
+
+
+def test_pcg32_matches_reference_rng(oracle):
+    with open(os.path.join(GOLDEN, "rng_ref.json")) as fh:
+        ref = json.load(fh)
+    assert ref["seed42_seq54"]["ints"][:6] == PCG32_DEMO_42_54
+    for key, state, seq in (("seed42_seq54", 42, 54), ("seed0", 0, None), ("seed7", 7, None)):
+        a = oracle.PCG32(state) if seq is None else oracle.PCG32(state, seq)
+        b = oracle.PCG32(state) if seq is None else oracle.PCG32(state, seq)
+        n = len(ref[key]["ints"])
+        assert [a.next_int() for _ in range(n)] == ref[key]["ints"]
+        assert [b.next_float() for _ in range(n)] == pytest.approx(ref[key]["floats"], abs=0, rel=1e-7)
+
+
+def test_next_float_is_top_23_bits(oracle):
+    a, b = oracle.PCG32(123, 9), oracle.PCG32(123, 9)
+    for _ in range(1000):
+        u = a.next_int()
+        f = b.next_float()
+        assert f == (u >> 9) * 2.0 ** -23
+        assert 0.0 <= f < 1.0
+
+
+# ---------------------------------------------------------------------------------------------
+# oracle films vs the reference's own StreakCameraFilm output
+
+def _scene_from_manifest(entry):
+    name = entry["scene"]
+    f, r = entry["film"], entry["render"]
+    if name == "cornell_box" or name == "box_rect_unwarp":
+        s = scenes.cornell_box(f["width"], f["height"], f["time_bins"], f["exposure"], r["sqrt_spp"],
+                               r["n_incoming"], r["max_bounces"], r["seed"])
+    elif name == "media_box":
+        s = scenes.media_box(f["width"], f["height"], f["scanline"], f["time_bins"], f["exposure"], r["sqrt_spp"],
+                             r["n_incoming"], r["max_bounces"], r["seed"])
+    elif name.startswith("icosphere"):
+        level = int(name[len("icosphere"):].split("_")[0])
+        s = scenes.icosphere_box(level, f["width"], f["height"], f["time_bins"], f["exposure"], r["sqrt_spp"],
+                                 r["n_incoming"], r["max_bounces"], r["seed"])
+    else:
+        raise KeyError(name)
+    s.point_lights = np.asarray(entry["point_lights"], dtype=np.float32).reshape(-1, 6)
+    s.rect_lights = np.asarray(entry["rect_lights"], dtype=np.float32).reshape(-1, 9)
+    s.film.camera_unwarp = bool(f["camera_unwarp"])
+    s.film.offset = float(f["offset"])
+    s.film.scanline = int(f["scanline"])
+    if entry["medium"] is not None:
+        s.medium = np.asarray(entry["medium"], dtype=np.float32)
+    import hashlib
+    assert hashlib.sha256(np.ascontiguousarray(s.tris).tobytes()).hexdigest() == entry["tris_sha256"]
+    return s
+
+
+MODES = {"render": "RNG_REFERENCE", "render_cs": "RNG_REFERENCE_PER_CS"}
+
+
+@pytest.mark.parametrize("tag", ["box_global", "media_cs", "ico3_cs", "rect_unwarp_cs"])
+def test_oracle_reproduces_reference_film(oracle, golden_manifest, tag):
+    entry = golden_manifest["films"][tag]
+    s = _scene_from_manifest(entry)
+    ref = np.load(os.path.join(GOLDEN, f"film_{tag}.npz"))["film"].astype(np.float64)
+    film, w, _ = oracle.render(s, getattr(oracle, MODES[entry["mode"]]), eps=oracle.NON_EMBREE_EPS)
+    mine = oracle.normalize(film, w)
+    # the reference normalises column x by the weight of pixel x+1 (StreakCameraFilm.h:373 reads
+    # channel 1 of a 1-channel image), which divides the last column by zero: compare the others
+    a, b = ref[:, :-1], mine[:, :-1]
+    assert np.isfinite(a).all()
+    per_pix = np.abs(a - b).sum(axis=(2, 3)) / np.maximum(np.abs(a).sum(axis=(2, 3)), 1e-3)
+    bad = int((per_pix > 1e-5).sum())
+    if entry["mode"] == "render_cs":
+        # the reference's non-Embree Triangle::intersect (plane projection, t from the plane
+        # equation) and the oracle's Moller test can disagree at shared edges or by one ulp of t;
+        # such a path changes, but reseeding per camera sample keeps it inside its pixel
+        assert bad <= 0.02 * per_pix.size
+        assert abs(b.sum() / a.sum() - 1) < 2e-3
+    else:
+        assert bad == 0
+    good = per_pix <= 1e-5
+    np.testing.assert_allclose(b[good], a[good], rtol=2e-5, atol=1e-6)
+
+
+def test_reference_last_column_defect_is_not_reproduced(oracle, golden_manifest):
+    ref = np.load(os.path.join(GOLDEN, "film_box_global.npz"))["film"]
+    last = ref[:, -1]
+    assert np.isinf(last).any()          # the reference's own output
+    s = _scene_from_manifest(golden_manifest["films"]["box_global"])
+    film, w, _ = oracle.render(s, oracle.RNG_REFERENCE, eps=oracle.NON_EMBREE_EPS)
+    mine = oracle.normalize(film, w)
+    assert np.isfinite(mine).all() and mine[:, -1].sum() > 0
+    # where the reference divided by the right weight, the last column equals sum / spp^2
+    np.testing.assert_allclose(mine[:, -1], film[:, -1] / w[:, -1, None, None])
+
+
+# ---------------------------------------------------------------------------------------------
+# the per-path RNG scheme (the GPU's) is statistically the reference's estimator
+
+def _zscores(oracle, tag, golden_manifest, k=16):
+    entry = golden_manifest["stats"][tag]
+    st = np.load(os.path.join(GOLDEN, f"stats_{tag}.npz"))
+    s = _scene_from_manifest(entry)
+    films = []
+    for seed in range(1000, 1000 + k):
+        f, w, _ = oracle.render(s, oracle.RNG_PER_PATH, seed=seed)
+        films.append(oracle.normalize(f, w))
+    f = np.stack(films)[:, :, :-1]
+    m_ref, s_ref = st["mean"][:, :-1].astype(np.float64), st["std"][:, :-1].astype(np.float64)
+    kr = int(st["k"])
+    m, sd = f.mean(0), f.std(0, ddof=1)
+    var = s_ref ** 2 / kr + sd ** 2 / k
+    nz = var > 0
+    z = (m[nz] - m_ref[nz]) / np.sqrt(var[nz])
+    # film total: its spread is estimated from this estimator's own per-seed totals
+    totals = f.sum(axis=(1, 2, 3, 4))
+    return z, float(st["mean"][:, :-1].sum()), totals, kr
+
+
+@pytest.mark.parametrize("tag", ["box", "media"])
+def test_per_path_rng_matches_reference_statistics(oracle, golden_manifest, tag):
+    z, total_ref, totals, kr = _zscores(oracle, tag, golden_manifest)
+    # multiple-testing rule: at most 1% of non-empty bins beyond 3 sigma, none beyond 8 sigma
+    assert np.mean(np.abs(z) > 3) <= 0.01
+    assert np.max(np.abs(z)) < 8
+    se = totals.std(ddof=1) * np.sqrt(1.0 / len(totals) + 1.0 / kr)
+    assert abs(totals.mean() - total_ref) < 4 * se
+
+
+def test_statistical_check_detects_a_dropped_cosine(oracle, golden_manifest):
+    """Mutation check of the statistical rule: scaling one Lambertian albedo by 0.8 (as a dropped
+    or doubled term would) must fail it."""
+    entry = golden_manifest["stats"]["box"]
+    st = np.load(os.path.join(GOLDEN, "stats_box.npz"))
+    s = _scene_from_manifest(entry)
+    s.materials = s.materials.copy()
+    s.materials[0] *= 0.8
+    f, w, _ = oracle.render(s, oracle.RNG_PER_PATH, seed=77)
+    pi = oracle.normalize(f, w)[:, :-1].sum()
+    assert abs(pi / st["mean"][:, :-1].sum() - 1) > 0.05
+
+
+# ---------------------------------------------------------------------------------------------
+# intersection
+
+def _near_edge(s, rays, ids, margin=1e-4):
+    """True where the hit lies within `margin` (barycentric) of an edge of its triangle."""
+    out = np.zeros(len(ids), dtype=bool)
+    for i, (r, k) in enumerate(zip(rays, ids)):
+        if k < 0:
+            continue
+        p0, p1, p2 = s.tris[k].astype(np.float64)
+        o, d = r[:3].astype(np.float64), r[3:].astype(np.float64)
+        e1, e2 = p1 - p0, p2 - p0
+        h = np.cross(d, e2)
+        a = np.dot(e1, h)
+        if abs(a) < 1e-12:
+            out[i] = True
+            continue
+        q = o - p0
+        u = np.dot(q, h) / a
+        v = np.dot(d, np.cross(q, e1)) / a
+        out[i] = min(u, v, 1 - u - v) < margin
+    return out
+
+
+@pytest.mark.parametrize("tag", ["box", "ico2"])
+def test_oracle_intersection_matches_reference_triangle_intersect(oracle, golden_manifest, tag):
+    entry = golden_manifest["isect"][tag]
+    g = np.load(os.path.join(GOLDEN, f"isect_{tag}.npz"))
+    s = scenes.cornell_box() if tag == "box" else scenes.icosphere_box(level=2)
+    rays, ref_ids, ref_t = g["rays"], g["ids"], g["t"]
+    # the reference's non-Embree Triangle::intersect rejects t < SMALLEST_DIST = 1e-4 (Ray.h:36)
+    ids, t = oracle.intersect(s, rays, tnear=1e-4, method=oracle.ISECT_MOLLER, use_bvh=True)
+    diff = ids != ref_ids
+    # excused: hits within 1e-4 (barycentric) of an edge, and ties (two triangles at the same
+    # distance, e.g. the inner box's bottom face on the floor), which the reference resolves by
+    # list order and the oracle to the lower index
+    tie = diff & (ids >= 0) & (ref_ids >= 0) & (np.abs(t - ref_t) <= 1e-6 * np.maximum(t, 1e-6))
+    excused = tie | _near_edge(s, rays, np.where(diff, np.maximum(ids, ref_ids), -1))
+    assert int((diff & ~excused).sum()) == 0
+    assert (diff & ~tie).sum() <= 0.002 * len(ids)
+    both = (~diff) & (ids >= 0)
+    # the reference takes t from the plane equation, which loses digits for short hits
+    np.testing.assert_allclose(t[both], ref_t[both], rtol=1e-5, atol=1e-6)
+    assert entry["n_rays"] == len(rays)
+
+
+@pytest.mark.parametrize("method", ["ISECT_MOLLER", "ISECT_WATERTIGHT"])
+def test_bvh_equals_brute_force(oracle, method):
+    s = scenes.icosphere_box(level=3)
+    rng = np.random.default_rng(5)
+    o = rng.uniform(0.05, 0.95, (3000, 3))
+    d = rng.normal(size=(3000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    m = getattr(oracle, method)
+    a_ids, a_t = oracle.intersect(s, rays, method=m, use_bvh=True)
+    b_ids, b_t = oracle.intersect(s, rays, method=m, use_bvh=False)
+    np.testing.assert_array_equal(a_ids, b_ids)
+    np.testing.assert_array_equal(a_t, b_t)
+
+
+def test_watertight_agrees_with_moller_away_from_edges(oracle):
+    s = scenes.icosphere_box(level=4)
+    rng = np.random.default_rng(8)
+    o = rng.uniform(0.05, 0.95, (4000, 3))
+    d = rng.normal(size=(4000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    m_ids, m_t = oracle.intersect(s, rays, method=oracle.ISECT_MOLLER)
+    w_ids, w_t = oracle.intersect(s, rays, method=oracle.ISECT_WATERTIGHT)
+    diff = m_ids != w_ids
+    excused = _near_edge(s, rays, np.where(diff, np.maximum(m_ids, w_ids), -1))
+    assert int((diff & ~excused).sum()) == 0
+    both = (~diff) & (m_ids >= 0)
+    # 1e-6 relative, plus ~1 ulp of the unit-box scale for hits a few 1e-4 from the origin
+    np.testing.assert_allclose(w_t[both], m_t[both], rtol=1e-6, atol=1e-7)
+
+
+def test_watertight_has_no_leaks_through_shared_edges_and_vertices(oracle):
+    """Rays from inside a closed icosphere through edge midpoints and vertices must hit it."""
+    verts, faces = scenes.icosphere_mesh(3)
+    c = np.array([0.5, 0.4, 0.5])
+    tris = (c + 0.3 * verts[faces[:, [0, 2, 1]]]).astype(np.float32)
+    s = scenes.Scene(tris=tris, tri_mat=np.zeros(len(tris), np.int32), materials=np.array([[0.5] * 3], np.float32),
+                     point_lights=np.zeros((0, 6), np.float32), rect_lights=np.zeros((0, 9), np.float32))
+    targets = np.concatenate([tris.astype(np.float64).reshape(-1, 3),
+                              (tris[:, [0, 1, 2]].astype(np.float64) + tris[:, [1, 2, 0]]).reshape(-1, 3) / 2])
+    d = targets - c
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([np.broadcast_to(c, d.shape), d], 1).astype(np.float32)
+    ids, t = oracle.intersect(s, rays, method=oracle.ISECT_WATERTIGHT)
+    assert (ids >= 0).all()
+
+
+def test_analytic_axis_aligned_hits(oracle):
+    s = scenes.cornell_box()
+    # straight down from (0.3, 0.9, 0.2) hits the floor (y=0) at t = 0.9; up hits the ceiling at 0.1
+    rays = np.array([[0.3, 0.9, 0.2, 0, -1, 0], [0.3, 0.9, 0.2, 0, 1, 0], [0.3, 0.9, 0.2, 0, 0, 1],
+                     [0.3, 0.9, 0.2, 0, 0, -1]], dtype=np.float32)
+    ids, t = oracle.intersect(s, rays)
+    np.testing.assert_allclose(t[:3], [0.9, 0.1, 0.8], rtol=1e-6)
+    assert ids[3] == -1  # the front of the box is open
+    assert s.tri_mat[ids[0]] == 0 and s.tris[ids[0]][:, 1].max() == 0.0
+
+
+# ---------------------------------------------------------------------------------------------
+# closed forms and invariants of the estimator
+
+def _floor_scene(light_h=0.8, kd=0.5, medium=None, sqrt_spp=8, width=8, time_bins=400, exposure=0.01):
+    """A single large floor quad (normal +y) under a point light, seen from above by a camera
+    looking straight down; one bounce."""
+    P = lambda x, z: [x, 0.0, z]
+    quad = np.array([[P(-4, -4), P(4, -4), P(4, 4)], [P(-4, -4), P(4, 4), P(-4, 4)]], dtype=np.float64)
+    tris = []
+    for t in quad:  # orient so that ng = cross(p2-p0, p1-p0) points +y
+        ng = np.cross(t[2] - t[0], t[1] - t[0])
+        tris.append(t if ng[1] > 0 else t[[0, 2, 1]])
+    s = scenes.Scene(tris=np.asarray(tris, np.float32), tri_mat=np.zeros(2, np.int32),
+                     materials=np.array([[kd] * 3], np.float32),
+                     point_lights=np.array([[0.5, light_h, 0.5, 1.0, 1.0, 1.0]], np.float32),
+                     rect_lights=np.zeros((0, 9), np.float32), medium=medium)
+    s.camera = scenes.Camera(position=(0.5, 1.0, 0.5), focus=(0.5, 0.0, 0.5), up=(0.0, 0.0, 1.0), fov_deg=30.0)
+    s.film = scenes.Film(width, width, time_bins, exposure)
+    s.render = scenes.RenderParams(sqrt_spp, 1, 1, 0)
+    return s
+
+
+def _expected_direct(s, n=64):
+    """Pixel-averaged direct radiance Kd/pi * I * cos / d^2 from independent ray/plane geometry."""
+    W = s.film.width
+    fov = math.radians(s.camera.fov_deg) / 2
+    fd = 1 / math.tan(fov)
+    c = np.array(s.camera.position)
+    nv = np.array(s.camera.focus) - c
+    nv /= np.linalg.norm(nv)
+    up = np.array(s.camera.up, float)
+    uv = np.cross(nv, up)
+    uv /= np.linalg.norm(uv)
+    vv = np.cross(uv, nv)
+    lp = s.point_lights[0, :3].astype(float)
+    I, kd = s.point_lights[0, 3], s.materials[0, 0]
+    out = np.zeros((W, W))
+    tmin = np.full((W, W), np.inf)
+    g = (np.arange(n) + 0.5) / n
+    for y in range(W):
+        for x in range(W):
+            px, py = np.meshgrid(x + g, y + g)
+            d = nv[None, None] * fd + 2 * uv * ((px - W / 2) / W)[..., None] - 2 * vv * ((py - W / 2) / W)[..., None]
+            d /= np.linalg.norm(d, axis=-1, keepdims=True)
+            t = -c[1] / d[..., 1]
+            p = c + d * t[..., None]
+            to_l = lp - p
+            dl = np.linalg.norm(to_l, axis=-1)
+            cos = to_l[..., 1] / dl
+            out[y, x] = np.mean(kd / math.pi * I * cos / dl ** 2)
+            tmin[y, x] = np.min(t + dl)
+    return out, tmin
+
+
+def test_direct_lighting_closed_form(oracle):
+    s = _floor_scene()
+    f, w, _ = oracle.render(s, oracle.RNG_PER_PATH)
+    img = oracle.normalize(f, w)
+    got = img.sum(axis=(2, 3)) / 3.0
+    exp, tmin = _expected_direct(s)
+    # stratified 8x8 jitter vs a 64x64 midpoint grid: smooth integrand, agreement to ~1e-3
+    np.testing.assert_allclose(got, exp, rtol=5e-3)
+    # every contribution arrives at its path length |c-x| + |x-l| (causality, exposure 0.01)
+    t_first = np.argmax(img.sum(axis=3) > 0, axis=2) * s.film.exposure
+    assert np.all(t_first >= np.floor(tmin / s.film.exposure) * s.film.exposure - 1e-6)
+
+
+def test_absorbing_medium_scales_direct_light_by_transmittance(oracle):
+    sa = 0.7
+    # 4096 camera samples per pixel; surviving the free flight is a coin with p ~ exp(-0.7) ~ 0.5,
+    # so one pixel carries ~1.6% and the 16-pixel total ~0.4% relative standard error
+    base = _floor_scene(sqrt_spp=64, width=4)
+    med = _floor_scene(sqrt_spp=64, width=4, medium=np.array([sa] * 3 + [0.0] * 3, np.float32))
+    f0, w0, _ = oracle.render(base, oracle.RNG_PER_PATH)
+    f1, w1, _ = oracle.render(med, oracle.RNG_PER_PATH, seed=3)
+    a, b = oracle.normalize(f0, w0)[..., 0], oracle.normalize(f1, w1)[..., 0]
+    T = a.shape[2]
+    tt = (np.arange(T) + 0.5) * base.film.exposure  # path length of each bin
+    # with sigma_s = 0 the estimator's expectation is the vacuum value times exp(-sigma_a * path length)
+    expected = (a * np.exp(-sa * tt)).sum(axis=2)
+    got = b.sum(axis=2)
+    assert np.all(np.abs(got / expected - 1) < 0.07)
+    assert abs(got.sum() / expected.sum() - 1) < 0.016
+
+
+def test_no_energy_before_the_straight_line_time(oracle):
+    s = scenes.cornell_box(width=16, height=16, time_bins=256, sqrt_spp=2, n_incoming=8, max_bounces=4)
+    f, w, _ = oracle.render(s, oracle.RNG_PER_PATH)
+    cam = np.array(s.camera.position)
+    light = s.point_lights[0, :3]
+    t_min = np.linalg.norm(cam - light)  # triangle inequality bound on every path length
+    first_bin = int(np.floor(t_min / s.film.exposure))
+    assert f[:, :, :first_bin].sum() == 0
+    assert f[:, :, first_bin:].sum() > 0
+
+
+def test_scanline_and_rows_tile_the_film(oracle):
+    s = scenes.cornell_box(width=8, height=8, time_bins=64, exposure=0.04, sqrt_spp=2, n_incoming=2)
+    full, wf, _ = oracle.render(s, oracle.RNG_PER_PATH)
+    top, wt, _ = oracle.render(s, oracle.RNG_PER_PATH, rows=(0, 3))
+    bot, wb, _ = oracle.render(s, oracle.RNG_PER_PATH, rows=(3, 8))
+    np.testing.assert_array_equal(np.concatenate([top, bot]), full)
+    s.film.scanline = 5
+    line, wl, _ = oracle.render(s, oracle.RNG_PER_PATH)
+    np.testing.assert_array_equal(line[0], full[5])
+    assert (wl == 4).all()
