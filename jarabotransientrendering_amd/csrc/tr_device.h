@@ -113,6 +113,9 @@ __device__ __forceinline__ RayPre ray_pre(f3 o, f3 d)
 // Returns the distance, or -1 when the ray misses the triangle's interior/edges.
 __device__ __forceinline__ float tri_watertight(const RayPre& r, f3 p0, f3 p1, f3 p2)
 {
+    // No fused multiply-adds here: an edge shared by two triangles must evaluate to exactly
+    // opposite values in both, which fma(Cx, By, -Cy*Bx) vs fma(Bx, Cy, -By*Cx) does not give.
+#pragma clang fp contract(off)
     f3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
     float Akx = comp(A, r.kx), Aky = comp(A, r.ky), Akz = comp(A, r.kz);
     float Bkx = comp(B, r.kx), Bky = comp(B, r.ky), Bkz = comp(B, r.kz);

@@ -39,7 +39,7 @@ using namespace trd;
 
 namespace {
 
-constexpr int kTraceBlock = 128;   // traversal kernels: 128 threads, kBvhStack ints of LDS each
+constexpr int kTraceBlock = 128;   // traversal kernels: 128 threads, STACK ints of LDS each
 constexpr int kBlock = 256;        // other kernels
 constexpr uint32_t kDefaultWave = 1u << 23;
 constexpr float kPi = 3.14159265358979323846f;
@@ -141,9 +141,10 @@ __global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S, RenderDev R,
 
 // ---------------------------------------------------------------------------------------------
 // extend: closest hit for each queued ray
+template <int STACK>
 __global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S, WaveDev Wv, int cur)
 {
-    __shared__ int32_t stack[kBvhStack * kTraceBlock];
+    __shared__ int32_t stack[STACK * kTraceBlock];
     const uint32_t count = Wv.counters[cur];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
@@ -406,9 +407,10 @@ __global__ void __launch_bounds__(kBlock) shade_kernel(SceneDev S, RenderDev R, 
 
 // ---------------------------------------------------------------------------------------------
 // shadow: occlusion test light -> vertex in (eps, dist - eps]; visible contributions go to the film
+template <int STACK>
 __global__ void __launch_bounds__(kTraceBlock) shadow_kernel(SceneDev S, RenderDev R, WaveDev Wv)
 {
-    __shared__ int32_t stack[kBvhStack * kTraceBlock];
+    __shared__ int32_t stack[STACK * kTraceBlock];
     const uint32_t count = Wv.counters[C_SHADOW];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0)
@@ -457,11 +459,12 @@ __global__ void __launch_bounds__(kBlock) film_merge_kernel(float4* __restrict__
 }
 
 // batch closest hit for the public query entry point
+template <int STACK>
 __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, const tr_ray* __restrict__ rays,
                                                                 tr_hit* __restrict__ hits, const uint2* __restrict__ ids,
                                                                 uint32_t n)
 {
-    __shared__ int32_t stack[kBvhStack * kTraceBlock];
+    __shared__ int32_t stack[STACK * kTraceBlock];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         tr_ray q = rays[i];
@@ -484,6 +487,16 @@ __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, cons
         hits[i] = h;
     }
 }
+
+// The traversal stack holds at most one entry per BVH level; the smallest instantiation that
+// covers the scene's depth keeps LDS per block (and so occupancy) in check.
+#define TR_STACK_DISPATCH(depth, KERNEL, GRID, BLOCK, STREAM, ...)                         \
+    do {                                                                                   \
+        if ((depth) < 16) KERNEL<16><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);             \
+        else if ((depth) < 24) KERNEL<24><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);        \
+        else if ((depth) < 32) KERNEL<32><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);        \
+        else KERNEL<kBvhStack><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);                   \
+    } while (0)
 
 }  // namespace
 
@@ -852,7 +865,7 @@ int tr_intersect_batch_device(tr_scene s, const tr_ray* d_rays, uint32_t n, tr_h
     if (n == 0) return TR_OK;
     if (!d_rays || !d_hits) return fail(TR_ERR_INVALID, "tr_intersect_batch_device: null buffer");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
-    intersect_kernel<<<trace_grid(n), kTraceBlock, 0, st>>>(s->view(), d_rays, d_hits, s->d_ids, n);
+    TR_STACK_DISPATCH(s->depth, intersect_kernel, trace_grid(n), kTraceBlock, st, s->view(), d_rays, d_hits, s->d_ids, n);
     HIP_TRY(hipGetLastError());
     return TR_OK;
 }
@@ -1112,13 +1125,13 @@ int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_re
         int cur = 0;
         for (int b = 0; b < bounces; ++b) {
             if ((rc = begin(TR_K_EXTEND))) return rc;
-            extend_kernel<<<tg, kTraceBlock, 0, s->stream>>>(S, Wv, cur);
+            TR_STACK_DISPATCH(s->depth, extend_kernel, tg, kTraceBlock, s->stream, S, Wv, cur);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADE))) return rc;
             shade_kernel<<<sg, kBlock, 0, s->stream>>>(S, R, Wv, cur);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADOW))) return rc;
-            shadow_kernel<<<tg, kTraceBlock, 0, s->stream>>>(S, R, Wv);
+            TR_STACK_DISPATCH(s->depth, shadow_kernel, tg, kTraceBlock, s->stream, S, R, Wv);
             if ((rc = end())) return rc;
             cur ^= 1;
         }

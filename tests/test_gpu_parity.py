@@ -43,16 +43,31 @@ def _near_edge(tris, o, d, ids, margin=1e-4):
     return out
 
 
+def _grazing(tris, d, ids, cos_min=0.05):
+    """Hits whose incidence cosine is below cos_min: t is ill-conditioned there (error ~ eps/cos)."""
+    out = np.zeros(len(ids), dtype=bool)
+    k = ids >= 0
+    p = tris[ids[k]].astype(np.float64)
+    n = np.cross(p[:, 2] - p[:, 0], p[:, 1] - p[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    out[k] = np.abs(np.sum(n * d[k], axis=1)) < cos_min
+    return out
+
+
 def _compare_ids(tris, o, d, ids, t, ref_ids, ref_t, max_frac=0.002):
-    """Closest-hit protocol: ids equal except hits within 1e-4 (barycentric) of an edge and exact
-    ties; distances within 1e-6 relative (plus ~1 ulp of the unit scale)."""
+    """Closest-hit protocol: ids equal except hits within 1e-4 (barycentric) of an edge and ties
+    (two hits within 1e-6 relative t); distances within 1e-6 relative (plus ~1 ulp of the unit
+    scale) except grazing hits (|cos| < 0.05, counted separately)."""
     diff = ids != ref_ids
     tie = diff & (ids >= 0) & (ref_ids >= 0) & (np.abs(t - ref_t) <= 1e-6 * np.maximum(np.abs(ref_t), 1e-6))
     edge = _near_edge(tris, o, d, np.where(diff, np.maximum(ids, ref_ids), -1))
     assert int((diff & ~(tie | edge)).sum()) == 0
     assert (diff & ~tie).sum() <= max_frac * len(ids)
     both = (~diff) & (ids >= 0)
-    np.testing.assert_allclose(t[both], ref_t[both], rtol=1e-6, atol=1e-7)
+    graze = _grazing(tris, d, np.where(both, ids, -1))
+    assert graze.sum() <= 0.02 * len(ids)
+    ok = both & ~graze
+    np.testing.assert_allclose(t[ok], ref_t[ok], rtol=1e-6, atol=1e-7)
 
 
 @pytest.mark.parametrize("level", [None, 3, 6])
@@ -64,9 +79,12 @@ def test_intersect_matches_oracle(R, oracle, level):
     ids = np.where(tri == 0xFFFFFFFF, -1, tri.astype(np.int64)).astype(np.int32)
     ref_ids, ref_t = oracle.intersect(s, np.concatenate([o, d], 1), method=oracle.ISECT_MOLLER)
     _compare_ids(s.tris, o, d, ids, t, ref_ids, ref_t)
-    # the watertight restatement agrees exactly on every id
+    # against the oracle's restatement of the same watertight test the kernel is bit-exact
+    # (same operations, no contraction); only exact-distance ties may resolve differently
     wt_ids, wt_t = oracle.intersect(s, np.concatenate([o, d], 1), method=oracle.ISECT_WATERTIGHT)
-    assert (ids == wt_ids).mean() > 0.999
+    same = ids == wt_ids
+    assert (same | (t == wt_t)).all()
+    np.testing.assert_array_equal(t, np.where(wt_ids >= 0, wt_t, -1.0).astype(np.float32))
 
 
 @pytest.mark.parametrize("tag", ["box", "ico2"])
@@ -137,7 +155,7 @@ def test_render_matches_oracle_same_streams(R, oracle, name):
     per_pix = np.abs(img - ref).sum(axis=(2, 3)) / np.maximum(np.abs(ref).sum(axis=(2, 3)), 1e-4)
     # identical random streams; float rounding (GPU libm and fused multiply-adds vs the oracle)
     # or an edge hit can send an occasional path elsewhere
-    assert (per_pix < 1e-4).mean() >= 0.97, np.sort(per_pix.ravel())[-10:]
+    assert (per_pix >= 1e-4).sum() <= max(1, 0.03 * per_pix.size), np.sort(per_pix.ravel())[-10:]
     assert abs(img.sum() / ref.sum() - 1) < 3e-3
     # film window bookkeeping: nothing outside [offset, offset + T*exposure)
     assert ost["contributions"] >= 0
