@@ -28,8 +28,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # algorithmic HBM bytes per unit of work (wavefront formulation, path state round-trips
 # through HBM between stages):
 BYTES_EXTEND_PER_RAY = 32 + 16          # ray in, hit out
-BYTES_SHADOW_PER_RAY = 32 + 4           # shadow ray in, occlusion out
-BYTES_FILM_PER_CONTRIB = 24             # FP32 RGB read-modify-write of the pass film
+BYTES_SHADOW_PER_RAY = 32 + 1           # shadow ray in, visibility byte out
+BYTES_FILM_PER_RECORD = 1               # visibility byte of every shadow record
+BYTES_FILM_PER_CONTRIB = 20 + 24        # record in (RGB, pixel, bin) + FP32 RGB read-modify-write
 BYTES_SHADE_PER_SEGMENT = 64 + 64 + 20  # path state in + out, NEE record out
 BYTES_PATH_PER_SEGMENT = 300            # whole segment: extension + shadow + state + record + film
 
@@ -115,6 +116,7 @@ def main():
     ap.add_argument("--wave", type=int, default=0, help="paths per wavefront (0 = library default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--binary-bvh", action="store_true", help="traverse the binary BVH (default: 4-wide)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -135,6 +137,7 @@ def main():
     eng = renderer.TransientRenderer()
     eng.wave_paths = args.wave
     eng.profile = True
+    eng.binary_bvh = args.binary_bvh
     for k in range(args.warmup):
         eng.render(None, world, integ, cam, film, sampler, seed=1000 + k)
     film.clean()
@@ -153,7 +156,8 @@ def main():
     con = sum(s["contributions"] for s in steps)
     alg_bytes = {
         "extend": BYTES_EXTEND_PER_RAY * ext,
-        "shadow": BYTES_SHADOW_PER_RAY * shd + BYTES_FILM_PER_CONTRIB * con,
+        "shadow": BYTES_SHADOW_PER_RAY * shd,
+        "film": BYTES_FILM_PER_RECORD * shd + BYTES_FILM_PER_CONTRIB * con,
         "shade": BYTES_SHADE_PER_SEGMENT * ext,
         "film_merge": 2 * 4 * film.rows * film.width * film.time_bins * 3 * args.steps,
         "camera": 80 * paths,
@@ -187,7 +191,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {desc}", "paths_per_step": paths // args.steps,
-                   "triangles": scene.n_tris, "parallelism": "single GPU"},
+                   "triangles": scene.n_tris, "bvh": "binary" if args.binary_bvh else "4-wide",
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": avg_ms, "launches": kern_n[dom],

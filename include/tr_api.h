@@ -156,11 +156,12 @@ typedef struct {
 } tr_render_desc;
 
 enum {
-    TR_RENDER_PROFILE = 1u  /* time each kernel with HIP events (tr_render_stats.kernel_ms) */
+    TR_RENDER_PROFILE = 1u,     /* time each kernel with HIP events (tr_render_stats.kernel_ms) */
+    TR_RENDER_BINARY_BVH = 2u   /* traverse the binary BVH instead of the default 4-wide one */
 };
 
 enum {
-    TR_K_CAMERA = 0, TR_K_EXTEND = 1, TR_K_SHADE = 2, TR_K_SHADOW = 3, TR_K_FILM_MERGE = 4,
+    TR_K_CAMERA = 0, TR_K_EXTEND = 1, TR_K_SHADE = 2, TR_K_SHADOW = 3, TR_K_FILM_MERGE = 4, TR_K_FILM = 5,
     TR_K_COUNT = 8
 };
 

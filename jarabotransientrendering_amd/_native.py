@@ -19,9 +19,10 @@ TR_ERR_NOMEM = -3
 TR_ERR_STATE = -4
 
 TR_RENDER_PROFILE = 1
-K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FILM_MERGE = 0, 1, 2, 3, 4
+TR_RENDER_BINARY_BVH = 2
+K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FILM_MERGE, K_FILM = 0, 1, 2, 3, 4, 5
 K_NAMES = {K_CAMERA: "camera", K_EXTEND: "extend", K_SHADE: "shade", K_SHADOW: "shadow",
-           K_FILM_MERGE: "film_merge"}
+           K_FILM_MERGE: "film_merge", K_FILM: "film"}
 TR_K_COUNT = 8
 
 

@@ -232,3 +232,87 @@ BvhBuildResult build_bvh(const float* tris, const int32_t* tri_mat, uint32_t n)
     r.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return r;
 }
+
+// ---- 4-wide collapse ------------------------------------------------------------------------
+namespace {
+
+struct ChildRef {
+    float lo[3], hi[3];
+    int32_t index, count;  // as in BvhNodeHost::d (count 0: inner, > 0: leaf, < 0: empty)
+    float area() const
+    {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return (dx >= 0.f) ? 2.f * (dx * dy + dy * dz + dz * dx) : 0.f;
+    }
+};
+
+ChildRef child_of(const BvhNodeHost& n, int side)
+{
+    ChildRef r;
+    const float* xy = side == 0 ? n.a : n.b;
+    r.lo[0] = xy[0];
+    r.hi[0] = xy[1];
+    r.lo[1] = xy[2];
+    r.hi[1] = xy[3];
+    r.lo[2] = n.c[side * 2 + 0];
+    r.hi[2] = n.c[side * 2 + 1];
+    r.index = n.d[side];
+    r.count = n.d[2 + side];
+    return r;
+}
+
+int32_t collapse(const std::vector<BvhNodeHost>& bin, int32_t bnode, std::vector<Bvh4NodeHost>& out, int depth,
+                 int* max_depth)
+{
+    ChildRef refs[4];
+    int n = 0;
+    refs[n++] = child_of(bin[bnode], 0);
+    refs[n++] = child_of(bin[bnode], 1);
+    while (n < 4) {
+        int best = -1;
+        float best_area = -1.f;
+        for (int k = 0; k < n; ++k)
+            if (refs[k].count == 0 && refs[k].area() > best_area) {
+                best_area = refs[k].area();
+                best = k;
+            }
+        if (best < 0) break;
+        const BvhNodeHost& open = bin[refs[best].index];
+        refs[best] = child_of(open, 0);
+        refs[n++] = child_of(open, 1);
+    }
+    int32_t me = int32_t(out.size());
+    out.emplace_back();
+    *max_depth = std::max(*max_depth, depth);
+    for (int k = 0; k < 4; ++k) {
+        const float inf = std::numeric_limits<float>::infinity();
+        bool used = k < n && refs[k].count >= 0;
+        out[me].lox[k] = used ? refs[k].lo[0] : inf;
+        out[me].hix[k] = used ? refs[k].hi[0] : -inf;
+        out[me].loy[k] = used ? refs[k].lo[1] : inf;
+        out[me].hiy[k] = used ? refs[k].hi[1] : -inf;
+        out[me].loz[k] = used ? refs[k].lo[2] : inf;
+        out[me].hiz[k] = used ? refs[k].hi[2] : -inf;
+        out[me].child[k] = used ? refs[k].index : 0;
+        out[me].count[k] = used ? refs[k].count : -1;
+    }
+    for (int k = 0; k < n; ++k)
+        if (refs[k].count == 0) {
+            int32_t c = collapse(bin, refs[k].index, out, depth + 1, max_depth);
+            out[me].child[k] = c;  // `out` may have grown: index, do not hold references
+        }
+    return me;
+}
+
+}  // namespace
+
+Bvh4 collapse_bvh4(const BvhBuildResult& b)
+{
+    Bvh4 r;
+    r.nodes.reserve(b.nodes.size() / 2 + 2);
+    int depth = 0;
+    collapse(b.nodes, 0, r.nodes, 1, &depth);
+    r.depth = depth;
+    r.stack_bound = 3 * depth;
+    return r;
+}

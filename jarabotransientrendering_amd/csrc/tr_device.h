@@ -70,6 +70,44 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
     return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
+// ---- block-aggregated queue append ----------------------------------------------------------
+// Every thread of the block calls this with its number of items (0 or more); the block issues one
+// atomic on `counter` and each thread gets the first queue position of its items. `smem` holds at
+// least blockDim.x / 64 + 1 words. Same-address atomics from every wavefront are the limiting
+// rate at tens of millions of queue entries, hence one per block.
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, uint32_t cnt, uint32_t* smem)
+{
+    const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    uint32_t x = cnt;  // inclusive scan within the wavefront
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) smem[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            uint32_t t = smem[w];
+            smem[w] = total;  // exclusive prefix of wavefront totals
+            total += t;
+        }
+        smem[nw] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos = smem[nw] + smem[wid] + x - cnt;
+    __syncthreads();  // smem may be reused by the caller's next iteration
+    return pos;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
 // ---- ray/box and watertight ray/triangle -----------------------------------------------------
 struct RayPre {
     f3 o, d, inv;
@@ -199,6 +237,77 @@ __device__ __forceinline__ int32_t trace(const float4* __restrict__ nodes, const
             node = dd.x;
         } else if (hr) {
             node = dd.y;
+        } else {
+            if (sp == 0) break;
+            --sp;
+            node = stack[sp * stride];
+        }
+    }
+    return hit;
+}
+
+// Same query over the 4-wide BVH (Bvh4NodeHost: 8 float4 per node). Children that the ray enters
+// are visited nearest first; the others are pushed far-to-near. Leaves are tested on the spot.
+template <bool ANY>
+__device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                          const RayPre& r, float tnear, float& tfar, int32_t* stack, int stride)
+{
+    int32_t hit = -1;
+    int32_t node = 0;
+    int sp = 0;
+    for (;;) {
+        const float4* n = nodes + 8 * size_t(node);
+        float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+        int4 ch = *reinterpret_cast<const int4*>(n + 6);
+        int4 cn = *reinterpret_cast<const int4*>(n + 7);
+        float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+        float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+        float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+        int cha[4] = {ch.x, ch.y, ch.z, ch.w}, cna[4] = {cn.x, cn.y, cn.z, cn.w};
+        float key[4];
+        int idx[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float x0 = (lxa[k] - r.o.x) * r.inv.x, x1 = (hxa[k] - r.o.x) * r.inv.x;
+            float y0 = (lya[k] - r.o.y) * r.inv.y, y1 = (hya[k] - r.o.y) * r.inv.y;
+            float z0 = (lza[k] - r.o.z) * r.inv.z, z1 = (hza[k] - r.o.z) * r.inv.z;
+            float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tnear));
+            float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar)) * 1.00000036f;
+            bool h = t0 <= t1 && cna[k] >= 0;
+            if (h && cna[k] > 0) {  // leaf: test its triangles now
+                for (int i = cha[k]; i < cha[k] + cna[k]; ++i) {
+                    const float4* t = tris + 3 * size_t(i);
+                    float4 q0 = t[0], q1 = t[1], q2 = t[2];
+                    float th = tri_watertight(r, mk(q0), mk(q1), mk(q2));
+                    int32_t id = __float_as_int(q0.w);
+                    if (th > tnear && th <= tfar && (th < tfar || hit < 0 || id < hit)) {
+                        hit = id;
+                        if (ANY) return hit;
+                        tfar = th;
+                    }
+                }
+                h = false;
+            }
+            key[k] = h ? t0 : __int_as_float(0x7f800000);
+            idx[k] = h ? cha[k] : -1;
+        }
+        // sort (key, idx) ascending: 5-comparator network
+#define TR_CSWAP(a, b)                                                            \
+    if (key[b] < key[a]) {                                                        \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk;                          \
+        int ti = idx[a]; idx[a] = idx[b]; idx[b] = ti;                            \
+    }
+        TR_CSWAP(0, 1) TR_CSWAP(2, 3) TR_CSWAP(0, 2) TR_CSWAP(1, 3) TR_CSWAP(1, 2)
+#undef TR_CSWAP
+        // push the farther hit children (far first), descend into the nearest
+#pragma unroll
+        for (int k = 3; k >= 1; --k)
+            if (idx[k] >= 0) {
+                stack[sp * stride] = idx[k];
+                ++sp;
+            }
+        if (idx[0] >= 0) {
+            node = idx[0];
         } else {
             if (sp == 0) break;
             --sp;

@@ -39,6 +39,26 @@ struct BvhBuildResult {
 // tris: n*9 floats (p0, p1, p2 as given), tri_mat: n material indices.
 BvhBuildResult build_bvh(const float* tris, const int32_t* tri_mat, uint32_t n);
 
+// ---- 4-wide BVH: one 128-byte node (one cache line of the GPU) holds four child boxes ---------
+//   lox, hix, loy, hiy, loz, hiz: child boxes, structure-of-arrays (6 x float4)
+//   child: node index (count == 0), first leaf-ordered triangle (count > 0); count -1: empty slot
+struct alignas(16) Bvh4NodeHost {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t child[4];
+    int32_t count[4];
+};
+static_assert(sizeof(Bvh4NodeHost) == 128, "node layout");
+
+struct Bvh4 {
+    std::vector<Bvh4NodeHost> nodes;  // root = node 0
+    int depth = 0;                     // levels of 4-wide nodes
+    int stack_bound = 0;               // entries a traversal stack may hold (3 per level)
+};
+
+// Collapses a binary BVH: every 4-wide node takes the two children of a binary node and opens the
+// inner child of largest surface area until it has four children or only leaves remain.
+Bvh4 collapse_bvh4(const BvhBuildResult& b);
+
 // ---- lights ------------------------------------------------------------------------------
 enum : int32_t { kLightPoint = 0, kLightRect = 1 };
 struct alignas(16) LightHost {

@@ -41,17 +41,35 @@ namespace {
 
 constexpr int kTraceBlock = 128;   // traversal kernels: 128 threads, STACK ints of LDS each
 constexpr int kBlock = 256;        // other kernels
-constexpr uint32_t kDefaultWave = 1u << 23;
+constexpr int kShadeBlock = 1024;  // shade: one queue atomic per 1024 paths
+constexpr uint32_t kDefaultWave = 1u << 25;
+constexpr uint32_t kFetchMax = 512;  // rays per work fetch of a traversal wavefront (upper bound)
+
+// Rays per work fetch: large enough that same-address fetch atomics stay rare, small enough that
+// every wavefront of the grid still gets several fetches when the queue is short.
+__device__ __forceinline__ uint32_t fetch_chunk(uint32_t count)
+{
+    uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    uint32_t c = count / (waves * 4u);
+    c = c < 64u ? 64u : (c > kFetchMax ? kFetchMax : c);
+    return c & ~63u;
+}
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInvPi = 0.318309886183790671538f;
 constexpr float kInv4Pi = 0.0795774715459476678844f;  // .25 * M_1_PI (IsotropicPhaseFunction.h:43-47)
 constexpr float kSigmaVisibility = 1e-5f;             // _SIGMA_VISIBILITY_ (bunnykiller.h:76-78)
 
 // counters[] layout (uint32 words)
-enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_STAT_EXT = 4, C_STAT_SHADOW = 6, C_STAT_CONTRIB = 8, C_WORDS = 16 };
+enum {
+    C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2,   // queue lengths
+    C_STAT_EXT = 4, C_STAT_SHADOW = 6, C_STAT_CONTRIB = 8,  // 64-bit statistics
+    C_FETCH_EXT = 10, C_FETCH_SHADOW = 11,              // work-fetch cursors of the persistent traversal grids
+    C_WORDS = 16
+};
 
 struct SceneDev {
-    const float4* nodes;
+    const float4* nodes;       // binary BVH (4 float4 per node)
+    const float4* nodes4;      // 4-wide BVH (8 float4 per node)
     const float4* tris;
     const float4* tri_normal;  // per scene triangle: shading normal xyz, w = material index (int bits)
     const float4* mats;        // Kd xyz, w = mean albedo used by the roulette
@@ -80,12 +98,13 @@ struct WaveDev {
     float4* ray_d;    // direction xyz, w = 0
     float4* thr;      // throughput xyz (with the pending BSDF/phase weight and roulette), w = time
     uint4* misc;      // x = path id lo, y = path id hi, z/w = path-stream PCG32 state
-    uint4* extra;     // x/y = light-stream PCG32 state, z = level, w = unwarp distance (float bits)
+    uint4* extra;     // x/y = light-stream PCG32 state, z = level | film pixel << 8, w = unwarp distance (float bits)
     float2* hit;      // x = distance, y = scene triangle (int bits) or -1
     uint32_t* q[2];
     float4* sh_o;     // light point xyz, w = distance
     float4* sh_d;     // direction xyz, w = film element index (uint bits)
-    float4* sh_c;     // contribution rgb
+    float4* sh_c;     // contribution rgb, w = film pixel (uint bits)
+    uint8_t* sh_vis;  // 1 if the shadow ray is unoccluded
     uint32_t* counters;
 };
 
@@ -93,20 +112,26 @@ __device__ __forceinline__ uint64_t u64(uint32_t lo, uint32_t hi) { return uint6
 
 // ---------------------------------------------------------------------------------------------
 // camera: one thread per path of the wavefront
-__global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S, RenderDev R, WaveDev Wv, uint64_t wave_base,
+__global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S, RenderDev R, WaveDev Wv, uint32_t wave_base,
                                                         uint32_t n)
 {
     uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot == 0) Wv.counters[C_Q0] = n;
+    if (slot == 0) {
+        Wv.counters[C_Q0] = n;
+        Wv.counters[C_FETCH_EXT] = 0;
+    }
     if (slot >= n) return;
-    uint64_t pid_local = wave_base + slot;
-    uint64_t cs_local = pid_local / uint64_t(R.n_inc);
-    uint32_t inc_i = uint32_t(pid_local - cs_local * uint64_t(R.n_inc));
-    uint64_t pix_local = cs_local / uint64_t(R.spp2);
-    uint32_t sub = uint32_t(cs_local - pix_local * uint64_t(R.spp2));
-    uint32_t xr = uint32_t(pix_local % uint64_t(R.W));
-    uint32_t y = uint32_t(pix_local / uint64_t(R.W)) + uint32_t(R.row0);
-    uint32_t sx = sub % uint32_t(R.spp), sy = sub / uint32_t(R.spp);
+    // a render call holds fewer than 2^32 paths: 32-bit index arithmetic
+    uint32_t pid_local = wave_base + slot;
+    uint32_t cs_local = pid_local / uint32_t(R.n_inc);
+    uint32_t inc_i = pid_local - cs_local * uint32_t(R.n_inc);
+    uint32_t pix_local = cs_local / uint32_t(R.spp2);   // film pixel (row relative)
+    uint32_t sub = cs_local - pix_local * uint32_t(R.spp2);
+    uint32_t rowl = pix_local / uint32_t(R.W);
+    uint32_t xr = pix_local - rowl * uint32_t(R.W);
+    uint32_t y = rowl + uint32_t(R.row0);
+    uint32_t sy = sub / uint32_t(R.spp);
+    uint32_t sx = sub - sy * uint32_t(R.spp);
     uint64_t cs = (uint64_t(y) * uint64_t(R.W) + xr) * uint64_t(R.spp2) + sub;
     uint64_t pid = cs * uint64_t(R.n_inc) + inc_i;
 
@@ -135,13 +160,13 @@ __global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S, RenderDev R,
     Wv.ray_d[slot] = make_float4(dir.x, dir.y, dir.z, 0.0f);
     Wv.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     Wv.misc[slot] = make_uint4(uint32_t(pid), uint32_t(pid >> 32), uint32_t(pst), uint32_t(pst >> 32));
-    Wv.extra[slot] = make_uint4(uint32_t(nst), uint32_t(nst >> 32), 0u, __float_as_uint(3.402823466e+38f));
+    Wv.extra[slot] = make_uint4(uint32_t(nst), uint32_t(nst >> 32), pix_local << 8, __float_as_uint(3.402823466e+38f));
     Wv.q[0][slot] = slot;
 }
 
 // ---------------------------------------------------------------------------------------------
 // extend: closest hit for each queued ray
-template <int STACK>
+template <bool WIDE, int STACK>
 __global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S, WaveDev Wv, int cur)
 {
     __shared__ int32_t stack[STACK * kTraceBlock];
@@ -150,16 +175,30 @@ __global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S, WaveDev
     if (gid == 0) {
         Wv.counters[cur ^ 1] = 0;
         Wv.counters[C_SHADOW] = 0;
+        Wv.counters[C_FETCH_SHADOW] = 0;
         atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_EXT), (unsigned long long)count);
     }
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = gid; i < count; i += stride) {
-        uint32_t slot = Wv.q[cur][i];
-        float4 o = Wv.ray_o[slot], d = Wv.ray_d[slot];
-        RayPre r = ray_pre(mk(o), mk(d));
-        float tfar = __int_as_float(0x7f800000);
-        int32_t tri = trace<false>(S.nodes, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock);
-        Wv.hit[slot] = make_float2(tri >= 0 ? tfar : -1.0f, __int_as_float(tri));
+    // persistent grid: each wavefront fetches kFetch queue entries at a time until the queue is
+    // drained (one fetch atomic per `chunk` rays)
+    const uint32_t lane = __lane_id();
+    const uint32_t chunk = fetch_chunk(count);
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(Wv.counters + C_FETCH_EXT, chunk);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        for (uint32_t j = 0; j < chunk; j += 64) {
+        const uint32_t i = base + j + lane;
+        if (i < count) {
+            uint32_t slot = Wv.q[cur][i];
+            float4 o = Wv.ray_o[slot], d = Wv.ray_d[slot];
+            RayPre r = ray_pre(mk(o), mk(d));
+            float tfar = __int_as_float(0x7f800000);
+            int32_t tri = WIDE ? trace4<false>(S.nodes4, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock)
+                               : trace<false>(S.nodes, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock);
+            Wv.hit[slot] = make_float2(tri >= 0 ? tfar : -1.0f, __int_as_float(tri));
+        }
+        }
     }
 }
 
@@ -228,9 +267,11 @@ __device__ __forceinline__ f3 sample_isotropic(f3 wi, float e1, float e2)
 
 // ---------------------------------------------------------------------------------------------
 // shade: one thread per queued path whose ray has just been traced
-__global__ void __launch_bounds__(kBlock) shade_kernel(SceneDev S, RenderDev R, WaveDev Wv, int cur)
+__global__ void __launch_bounds__(kShadeBlock) shade_kernel(SceneDev S, RenderDev R, WaveDev Wv, int cur)
 {
+    __shared__ uint32_t scan[kShadeBlock / 64 + 1];
     const uint32_t count = Wv.counters[cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) Wv.counters[C_FETCH_EXT] = 0;  // for the next extend
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
@@ -245,7 +286,7 @@ __global__ void __launch_bounds__(kBlock) shade_kernel(SceneDev S, RenderDev R, 
             uint4 misc = Wv.misc[slot], extra = Wv.extra[slot];
             float2 h = Wv.hit[slot];
             int32_t tri = __float_as_int(h.y);
-            uint32_t level = extra.z;
+            uint32_t level = extra.z & 0xFFu;
             f3 o = mk(o4), d = mk(d4);
             float sampled = o4.w;
             if (level == 0 && R.unwarp) extra.w = __float_as_uint(tri >= 0 ? h.x : 3.402823466e+38f);
@@ -335,15 +376,12 @@ __global__ void __launch_bounds__(kBlock) shade_kernel(SceneDev S, RenderDev R, 
                         bool finite = isfinite(val.x) && isfinite(val.y) && isfinite(val.z);
                         bool nonzero = val.x != 0.0f || val.y != 0.0f || val.z != 0.0f;
                         if (sensor >= 0.0f && sensor < float(R.T) && finite && nonzero) {
-                            uint32_t pix = uint32_t(pid / uint64_t(R.n_inc * R.spp2));  // (y*W + x)
-                            uint32_t rowl = pix / uint32_t(R.W) - uint32_t(R.row0);
-                            uint32_t x = pix % uint32_t(R.W);
                             uint32_t bin = min(uint32_t(floorf(sensor)), uint32_t(R.T - 1));
-                            uint32_t fidx = (rowl * uint32_t(R.W) + x) * uint32_t(R.T) + bin;
+                            uint32_t fidx = (extra.z >> 8) * uint32_t(R.T) + bin;
                             emit_shadow = true;
                             sh_o = make_float4(lp.x, lp.y, lp.z, ldist);
                             sh_d = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(fidx));
-                            sh_c = make_float4(val.x, val.y, val.z, 0.0f);
+                            sh_c = make_float4(val.x, val.y, val.z, __uint_as_float(extra.z >> 8));
                         }
                     }
                 }
@@ -382,18 +420,18 @@ __global__ void __launch_bounds__(kBlock) shade_kernel(SceneDev S, RenderDev R, 
                         n_d = make_float4(nd.x, nd.y, nd.z, 0.0f);
                         n_thr = make_float4(nf.x, nf.y, nf.z, time);
                         n_misc = make_uint4(misc.x, misc.y, uint32_t(pst), uint32_t(pst >> 32));
-                        n_extra = make_uint4(uint32_t(nst), uint32_t(nst >> 32), level + 1, extra.w);
+                        n_extra = make_uint4(uint32_t(nst), uint32_t(nst >> 32), extra.z + 1u, extra.w);
                     }
                 }
             }
         }
-        uint32_t sh_pos = wave_append(Wv.counters + C_SHADOW, emit_shadow);
+        uint32_t sh_pos = block_append(Wv.counters + C_SHADOW, emit_shadow ? 1u : 0u, scan);
         if (emit_shadow) {
             Wv.sh_o[sh_pos] = sh_o;
             Wv.sh_d[sh_pos] = sh_d;
             Wv.sh_c[sh_pos] = sh_c;
         }
-        uint32_t q_pos = wave_append(Wv.counters + (cur ^ 1), cont);
+        uint32_t q_pos = block_append(Wv.counters + (cur ^ 1), cont ? 1u : 0u, scan);
         if (cont) {
             Wv.ray_o[slot] = n_o;
             Wv.ray_d[slot] = n_d;
@@ -407,31 +445,98 @@ __global__ void __launch_bounds__(kBlock) shade_kernel(SceneDev S, RenderDev R, 
 
 // ---------------------------------------------------------------------------------------------
 // shadow: occlusion test light -> vertex in (eps, dist - eps]; visible contributions go to the film
-template <int STACK>
-__global__ void __launch_bounds__(kTraceBlock) shadow_kernel(SceneDev S, RenderDev R, WaveDev Wv)
+template <bool WIDE, int STACK>
+__global__ void __launch_bounds__(kTraceBlock) shadow_kernel(SceneDev S, WaveDev Wv)
 {
     __shared__ int32_t stack[STACK * kTraceBlock];
     const uint32_t count = Wv.counters[C_SHADOW];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0)
         atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_SHADOW), (unsigned long long)count);
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t lane = __lane_id();
     uint32_t visible = 0;
-    for (uint32_t i = gid; i < count; i += stride) {
-        float4 o = Wv.sh_o[i], d = Wv.sh_d[i];
-        RayPre r = ray_pre(mk(o), mk(d));
-        float tfar = o.w - kSigmaVisibility;
-        int32_t occ = trace<true>(S.nodes, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock);
-        if (occ < 0) {
-            float4 c = Wv.sh_c[i];
-            float* dst = R.film + size_t(__float_as_uint(d.w)) * 3;
-            unsafeAtomicAdd(dst + 0, c.x);
-            unsafeAtomicAdd(dst + 1, c.y);
-            unsafeAtomicAdd(dst + 2, c.z);
-            ++visible;
+    const uint32_t chunk = fetch_chunk(count);
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(Wv.counters + C_FETCH_SHADOW, chunk);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        for (uint32_t j = 0; j < chunk; j += 64) {
+            const uint32_t i = base + j + lane;
+            if (i < count) {
+                float4 o = Wv.sh_o[i], d = Wv.sh_d[i];
+                RayPre r = ray_pre(mk(o), mk(d));
+                float tfar = o.w - kSigmaVisibility;
+                bool vis = (WIDE ? trace4<true>(S.nodes4, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock)
+                                 : trace<true>(S.nodes, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock)) < 0;
+                Wv.sh_vis[i] = vis ? 1 : 0;
+                visible += vis ? 1u : 0u;
+            }
         }
     }
-    if (visible) atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_CONTRIB), (unsigned long long)visible);
+    visible = wave_sum(visible);
+    if (lane == 0 && visible)
+        atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_CONTRIB), (unsigned long long)visible);
+}
+
+// ---------------------------------------------------------------------------------------------
+// film: bins the visible contributions of this bounce into the pass film. A block takes a range
+// of kFilmRange consecutive shadow records; their pixels are few (paths are enumerated pixel-major
+// and queue compaction keeps runs of neighbouring paths together), so each block accumulates into
+// LDS histograms of up to `pix_slots` pixels x T bins x RGB and flushes one float atomic per
+// non-zero bin; records of further pixels go straight to HBM atomics.
+// (StreakCameraFilm::add_samples / add_temporal_sample, StreakCameraFilm.h:139-192, 280-297)
+constexpr int kFilmBlock = 256;
+constexpr uint32_t kFilmRange = 4096;
+constexpr uint32_t kFilmLdsBytes = 48 * 1024;
+constexpr uint32_t kFilmMaxSlots = 8;
+
+__global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev Wv, uint32_t pix_slots)
+{
+    extern __shared__ float hist[];          // pix_slots * T * 3
+    __shared__ uint32_t slot_pix[kFilmMaxSlots];
+    const uint32_t count = Wv.counters[C_SHADOW];
+    const uint32_t T = uint32_t(R.T), T3 = 3u * T, hsize = pix_slots * T3;
+    for (uint32_t start = blockIdx.x * kFilmRange; start < count; start += gridDim.x * kFilmRange) {
+        const uint32_t end = min(start + kFilmRange, count);
+        for (uint32_t i = threadIdx.x; i < hsize; i += blockDim.x) hist[i] = 0.0f;
+        if (threadIdx.x < kFilmMaxSlots) slot_pix[threadIdx.x] = 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t r = start + threadIdx.x; r < end; r += blockDim.x) {
+            if (!Wv.sh_vis[r]) continue;
+            float4 c = Wv.sh_c[r];
+            uint32_t pix = __float_as_uint(c.w);
+            uint32_t bin = __float_as_uint(Wv.sh_d[r].w) - pix * T;
+            int slot = -1;
+            for (uint32_t k = 0; k < pix_slots; ++k) {
+                uint32_t v = atomicCAS(&slot_pix[k], 0xFFFFFFFFu, pix);
+                if (v == 0xFFFFFFFFu || v == pix) {
+                    slot = int(k);
+                    break;
+                }
+            }
+            if (slot >= 0) {
+                float* h = hist + uint32_t(slot) * T3 + bin * 3u;
+                atomicAdd(h + 0, c.x);
+                atomicAdd(h + 1, c.y);
+                atomicAdd(h + 2, c.z);
+            } else {
+                float* dst = R.film + (size_t(pix) * T + bin) * 3;
+                unsafeAtomicAdd(dst + 0, c.x);
+                unsafeAtomicAdd(dst + 1, c.y);
+                unsafeAtomicAdd(dst + 2, c.z);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < hsize; i += blockDim.x) {
+            float v = hist[i];
+            if (v != 0.0f) {
+                uint32_t k = i / T3;
+                unsafeAtomicAdd(R.film + size_t(slot_pix[k]) * T3 + (i - k * T3), v);
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -459,7 +564,7 @@ __global__ void __launch_bounds__(kBlock) film_merge_kernel(float4* __restrict__
 }
 
 // batch closest hit for the public query entry point
-template <int STACK>
+template <bool WIDE, int STACK>
 __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, const tr_ray* __restrict__ rays,
                                                                 tr_hit* __restrict__ hits, const uint2* __restrict__ ids,
                                                                 uint32_t n)
@@ -470,7 +575,8 @@ __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, cons
         tr_ray q = rays[i];
         RayPre r = ray_pre(mk(q.org[0], q.org[1], q.org[2]), mk(q.dir[0], q.dir[1], q.dir[2]));
         float tfar = q.tfar;
-        int32_t tri = trace<false>(S.nodes, S.tris, r, q.tnear, tfar, stack + threadIdx.x, kTraceBlock);
+        int32_t tri = WIDE ? trace4<false>(S.nodes4, S.tris, r, q.tnear, tfar, stack + threadIdx.x, kTraceBlock)
+                           : trace<false>(S.nodes, S.tris, r, q.tnear, tfar, stack + threadIdx.x, kTraceBlock);
         tr_hit h;
         if (tri >= 0) {
             uint2 gp = ids[tri];
@@ -490,12 +596,18 @@ __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, cons
 
 // The traversal stack holds at most one entry per BVH level; the smallest instantiation that
 // covers the scene's depth keeps LDS per block (and so occupancy) in check.
-#define TR_STACK_DISPATCH(depth, KERNEL, GRID, BLOCK, STREAM, ...)                         \
+// `bound` = entries the traversal stack can hold at most (binary: depth; 4-wide: 3 per level).
+#define TR_STACK_DISPATCH_W(W, bound, KERNEL, GRID, BLOCK, STREAM, ...)                   \
     do {                                                                                   \
-        if ((depth) < 16) KERNEL<16><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);             \
-        else if ((depth) < 24) KERNEL<24><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);        \
-        else if ((depth) < 32) KERNEL<32><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);        \
-        else KERNEL<kBvhStack><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);                   \
+        if ((bound) < 16) KERNEL<W, 16><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);          \
+        else if ((bound) < 24) KERNEL<W, 24><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);     \
+        else if ((bound) < 32) KERNEL<W, 32><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);     \
+        else KERNEL<W, kBvhStack><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);                \
+    } while (0)
+#define TR_STACK_DISPATCH(wide, s, KERNEL, GRID, BLOCK, STREAM, ...)                       \
+    do {                                                                                   \
+        if (wide) TR_STACK_DISPATCH_W(true, (s)->stack4, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);  \
+        else TR_STACK_DISPATCH_W(false, (s)->depth, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);       \
     } while (0)
 
 }  // namespace
@@ -558,12 +670,13 @@ struct WaveBuffers {
     float2* hit = nullptr;
     uint32_t *q0 = nullptr, *q1 = nullptr;
     float4 *sh_o = nullptr, *sh_d = nullptr, *sh_c = nullptr;
+    uint8_t* sh_vis = nullptr;
     uint32_t* counters = nullptr;
 
     void release()
     {
         dev_free(ray_o); dev_free(ray_d); dev_free(thr); dev_free(misc); dev_free(extra); dev_free(hit);
-        dev_free(q0); dev_free(q1); dev_free(sh_o); dev_free(sh_d); dev_free(sh_c); dev_free(counters);
+        dev_free(q0); dev_free(q1); dev_free(sh_o); dev_free(sh_d); dev_free(sh_c); dev_free(sh_vis); dev_free(counters);
         cap = 0;
     }
     int ensure(uint32_t n)
@@ -574,7 +687,8 @@ struct WaveBuffers {
         if ((rc = dev_alloc(&ray_o, n)) || (rc = dev_alloc(&ray_d, n)) || (rc = dev_alloc(&thr, n)) ||
             (rc = dev_alloc(&misc, n)) || (rc = dev_alloc(&extra, n)) || (rc = dev_alloc(&hit, n)) ||
             (rc = dev_alloc(&q0, n)) || (rc = dev_alloc(&q1, n)) || (rc = dev_alloc(&sh_o, n)) ||
-            (rc = dev_alloc(&sh_d, n)) || (rc = dev_alloc(&sh_c, n)) || (rc = dev_alloc(&counters, C_WORDS))) {
+            (rc = dev_alloc(&sh_d, n)) || (rc = dev_alloc(&sh_c, n)) || (rc = dev_alloc(&sh_vis, n)) ||
+            (rc = dev_alloc(&counters, C_WORDS))) {
             release();
             return rc;
         }
@@ -585,7 +699,7 @@ struct WaveBuffers {
     {
         WaveDev w;
         w.ray_o = ray_o; w.ray_d = ray_d; w.thr = thr; w.misc = misc; w.extra = extra; w.hit = hit;
-        w.q[0] = q0; w.q[1] = q1; w.sh_o = sh_o; w.sh_d = sh_d; w.sh_c = sh_c; w.counters = counters;
+        w.q[0] = q0; w.q[1] = q1; w.sh_o = sh_o; w.sh_d = sh_d; w.sh_c = sh_c; w.sh_vis = sh_vis; w.counters = counters;
         return w;
     }
 };
@@ -603,20 +717,22 @@ struct tr_scene_s {
     float sa[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
     bool committed = false;
     hipStream_t stream = nullptr;
-    float4 *d_nodes = nullptr, *d_tris = nullptr, *d_tri_normal = nullptr, *d_mats = nullptr;
+    float4 *d_nodes = nullptr, *d_nodes4 = nullptr, *d_tris = nullptr, *d_tri_normal = nullptr, *d_mats = nullptr;
     LightHost* d_lights = nullptr;
     uint2* d_ids = nullptr;
-    uint32_t n_nodes = 0;
-    int depth = 0;
+    uint32_t n_nodes = 0, n_nodes4 = 0;
+    int depth = 0, depth4 = 0, stack4 = 0;
+    bool wide = true;   // traverse the 4-wide BVH (tr_render_desc flags can select the binary one)
     double build_ms = 0.0;
     uint64_t dev_bytes = 0;
     WaveBuffers wave;
     hipEvent_t ev[2 * 256] = {};
     bool ev_made = false;
+    int compute_units = 256;
 
     void release_device()
     {
-        dev_free(d_nodes); dev_free(d_tris); dev_free(d_tri_normal); dev_free(d_mats); dev_free(d_lights);
+        dev_free(d_nodes); dev_free(d_nodes4); dev_free(d_tris); dev_free(d_tri_normal); dev_free(d_mats); dev_free(d_lights);
         dev_free(d_ids);
         dev_bytes = 0;
     }
@@ -624,6 +740,7 @@ struct tr_scene_s {
     {
         SceneDev s;
         s.nodes = d_nodes;
+        s.nodes4 = d_nodes4;
         s.tris = d_tris;
         s.tri_normal = d_tri_normal;
         s.mats = d_mats;
@@ -676,6 +793,12 @@ int tr_scene_create(tr_scene* out)
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         delete s;
         return fail(TR_ERR_DEVICE, "tr_scene_create: hipStreamCreate failed");
+    }
+    {
+        int dev = 0;
+        hipDeviceProp_t p;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+            s->compute_units = p.multiProcessorCount;
     }
     *out = s;
     return TR_OK;
@@ -798,6 +921,8 @@ int tr_scene_commit(tr_scene s)
     uint32_t n = uint32_t(s->tri_mat.size());
     BvhBuildResult b = build_bvh(s->tris.data(), s->tri_mat.data(), n);
     if (b.depth >= kBvhStack) return fail(TR_ERR_INVALID, "tr_scene_commit: BVH deeper than the traversal stack");
+    Bvh4 b4 = collapse_bvh4(b);
+    if (b4.stack_bound >= kBvhStack) return fail(TR_ERR_INVALID, "tr_scene_commit: 4-wide BVH deeper than the traversal stack");
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->release_device();
     std::vector<float4> normals(n > 0 ? n : 1);
@@ -818,13 +943,15 @@ int tr_scene_commit(tr_scene s)
     size_t nn = b.nodes.size(), nt = b.tris.size(), nm = std::max<size_t>(1, s->mats.size() / 4),
            nl = std::max<size_t>(1, s->lights.size());
     int rc;
-    if ((rc = dev_alloc(&s->d_nodes, 4 * nn, &s->dev_bytes)) || (rc = dev_alloc(&s->d_tris, 3 * std::max<size_t>(1, nt), &s->dev_bytes)) ||
+    if ((rc = dev_alloc(&s->d_nodes, 4 * nn, &s->dev_bytes)) || (rc = dev_alloc(&s->d_nodes4, 8 * b4.nodes.size(), &s->dev_bytes)) || (rc = dev_alloc(&s->d_tris, 3 * std::max<size_t>(1, nt), &s->dev_bytes)) ||
         (rc = dev_alloc(&s->d_tri_normal, normals.size(), &s->dev_bytes)) || (rc = dev_alloc(&s->d_mats, nm, &s->dev_bytes)) ||
         (rc = dev_alloc(&s->d_lights, nl, &s->dev_bytes)) || (rc = dev_alloc(&s->d_ids, ids.size(), &s->dev_bytes))) {
         s->release_device();
         return rc;
     }
     HIP_TRY(hipMemcpyAsync(s->d_nodes, b.nodes.data(), nn * sizeof(BvhNodeHost), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_nodes4, b4.nodes.data(), b4.nodes.size() * sizeof(Bvh4NodeHost), hipMemcpyHostToDevice,
+                           s->stream));
     if (nt) HIP_TRY(hipMemcpyAsync(s->d_tris, b.tris.data(), nt * sizeof(TriHost), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_tri_normal, normals.data(), normals.size() * sizeof(float4), hipMemcpyHostToDevice, s->stream));
     if (!s->mats.empty())
@@ -835,6 +962,9 @@ int tr_scene_commit(tr_scene s)
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->n_nodes = uint32_t(nn);
     s->depth = b.depth;
+    s->n_nodes4 = uint32_t(b4.nodes.size());
+    s->depth4 = b4.depth;
+    s->stack4 = b4.stack_bound;
     s->build_ms = b.build_ms;
     s->committed = true;
     return TR_OK;
@@ -858,6 +988,14 @@ static uint32_t trace_grid(uint32_t n)
     return std::max(1u, std::min(blocks, 256u * 40u));
 }
 
+// Persistent traversal grids: enough blocks to fill every CU (each wavefront then keeps fetching
+// queue entries), never more blocks than the wavefront has rays.
+static uint32_t persistent_grid(uint32_t n, int compute_units)
+{
+    uint32_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
+    return std::max(1u, std::min(blocks, uint32_t(std::max(compute_units, 1)) * 16u));
+}
+
 int tr_intersect_batch_device(tr_scene s, const tr_ray* d_rays, uint32_t n, tr_hit* d_hits, void* stream)
 {
     if (!s) return fail(TR_ERR_INVALID, "tr_intersect_batch_device: null scene");
@@ -865,7 +1003,7 @@ int tr_intersect_batch_device(tr_scene s, const tr_ray* d_rays, uint32_t n, tr_h
     if (n == 0) return TR_OK;
     if (!d_rays || !d_hits) return fail(TR_ERR_INVALID, "tr_intersect_batch_device: null buffer");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
-    TR_STACK_DISPATCH(s->depth, intersect_kernel, trace_grid(n), kTraceBlock, st, s->view(), d_rays, d_hits, s->d_ids, n);
+    TR_STACK_DISPATCH(s->wide, s, intersect_kernel, trace_grid(n), kTraceBlock, st, s->view(), d_rays, d_hits, s->d_ids, n);
     HIP_TRY(hipGetLastError());
     return TR_OK;
 }
@@ -1016,6 +1154,11 @@ int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_re
     const uint64_t total = uint64_t(f->rows) * uint64_t(fd.width) * spp2 * uint64_t(d->n_incoming);
     if ((uint64_t(fd.height) * fd.width * spp2 * uint64_t(d->n_incoming)) >= (1ull << 61))
         return fail(TR_ERR_INVALID, "tr_render_transient: too many paths for the RNG stream ids");
+    if (total >= (1ull << 32))
+        return fail(TR_ERR_INVALID, "tr_render_transient: more than 2^32 - 1 paths in one call; split into passes");
+    if (uint64_t(f->rows) * uint64_t(fd.width) >= (1ull << 24))
+        return fail(TR_ERR_INVALID, "tr_render_transient: film has more than 2^24 pixels");
+    if (d->max_bounces > 255) return fail(TR_ERR_INVALID, "tr_render_transient: max_bounces above 255");
     uint32_t wave = d->wave_paths ? d->wave_paths : kDefaultWave;
     wave = uint32_t(std::min<uint64_t>(wave, std::max<uint64_t>(total, 1)));
     int rc = s->wave.ensure(wave);
@@ -1069,6 +1212,7 @@ int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_re
     R.seed = d->seed;
 
     const bool prof = (d->flags & TR_RENDER_PROFILE) != 0;
+    const bool wide = (d->flags & TR_RENDER_BINARY_BVH) == 0;
     if (prof && !s->ev_made) {
         for (auto& e : s->ev) HIP_TRY(hipEventCreate(&e));
         s->ev_made = true;
@@ -1112,26 +1256,33 @@ int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_re
     // the pass buffer is written on the scene stream; order it after any pending film work
     HIP_TRY(hipStreamSynchronize(f->stream));
 
+    // LDS histogram slots of the film kernel: as many pixels x T bins x RGB as fit in kFilmLdsBytes
+    const uint32_t pix_slots = std::min<uint32_t>(kFilmMaxSlots, kFilmLdsBytes / (uint32_t(fd.time_bins) * 12u));
+    const size_t film_lds = size_t(pix_slots) * size_t(fd.time_bins) * 12u;
     uint32_t n_waves = 0;
     int bounces = d->max_bounces;
     for (uint64_t base = 0; base < total; base += wave) {
         uint32_t n = uint32_t(std::min<uint64_t>(wave, total - base));
         ++n_waves;
         if ((rc = begin(TR_K_CAMERA))) return rc;
-        camera_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(S, R, Wv, base, n);
+        camera_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(S, R, Wv, uint32_t(base), n);
         if ((rc = end())) return rc;
-        uint32_t tg = trace_grid(n);
-        uint32_t sg = std::max(1u, std::min((n + kBlock - 1) / kBlock, 256u * 16u));
+        uint32_t tg = persistent_grid(n, s->compute_units);
+        uint32_t fg = std::max(1u, std::min((n + kFilmRange - 1) / kFilmRange, uint32_t(std::max(s->compute_units, 1)) * 4u));
+        uint32_t sg = std::max(1u, std::min((n + kShadeBlock - 1) / kShadeBlock, uint32_t(std::max(s->compute_units, 1)) * 2u));
         int cur = 0;
         for (int b = 0; b < bounces; ++b) {
             if ((rc = begin(TR_K_EXTEND))) return rc;
-            TR_STACK_DISPATCH(s->depth, extend_kernel, tg, kTraceBlock, s->stream, S, Wv, cur);
+            TR_STACK_DISPATCH(wide, s, extend_kernel, tg, kTraceBlock, s->stream, S, Wv, cur);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADE))) return rc;
-            shade_kernel<<<sg, kBlock, 0, s->stream>>>(S, R, Wv, cur);
+            shade_kernel<<<sg, kShadeBlock, 0, s->stream>>>(S, R, Wv, cur);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADOW))) return rc;
-            TR_STACK_DISPATCH(s->depth, shadow_kernel, tg, kTraceBlock, s->stream, S, R, Wv);
+            TR_STACK_DISPATCH(wide, s, shadow_kernel, tg, kTraceBlock, s->stream, S, Wv);
+            if ((rc = end())) return rc;
+            if ((rc = begin(TR_K_FILM))) return rc;
+            film_kernel<<<fg, kFilmBlock, film_lds, s->stream>>>(R, Wv, pix_slots);
             if ((rc = end())) return rc;
             cur ^= 1;
         }

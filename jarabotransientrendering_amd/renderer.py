@@ -319,6 +319,7 @@ class TransientRenderer:
             raise NotImplementedError("transient time-sampling mode is not part of this path")
         self.wave_paths = 0
         self.profile = False
+        self.binary_bvh = False   # traverse the binary BVH instead of the 4-wide one
         self.last_stats: Optional[dict] = None
 
     def render(self, name: Optional[str], world: World, integrator: Optional[BidirectionalPathTracing],
@@ -332,8 +333,9 @@ class TransientRenderer:
         if (sampler.size_x, sampler.size_y) != (film.width, film.height):
             raise ValueError("sampler and film sizes differ")
         integrator.preprocess()
+        flags = (nat.TR_RENDER_PROFILE if self.profile else 0) | (nat.TR_RENDER_BINARY_BVH if self.binary_bvh else 0)
         d = nat.RenderDesc(sampler.sqrt_spp, integrator.incoming_samples, integrator.max_path_size, 0, int(seed),
-                           int(self.wave_paths), nat.TR_RENDER_PROFILE if self.profile else 0)
+                           int(self.wave_paths), flags)
         st = nat.RenderStats()
         cam = camera.desc()
         nat.check(nat.lib().tr_render_transient(world.handle, ctypes.byref(cam), film.handle, ctypes.byref(d),
@@ -378,11 +380,13 @@ def setup_scene(scene):
     return w, cam, sampler, film, integ
 
 
-def render_scene(scene, seed: Optional[int] = None, wave_paths: int = 0, profile: bool = False):
+def render_scene(scene, seed: Optional[int] = None, wave_paths: int = 0, profile: bool = False,
+                 binary_bvh: bool = False):
     """Render a `scenes.Scene`; returns (normalised film [rows][W][T][3], stats, film object)."""
     w, cam, sampler, film, integ = setup_scene(scene)
     eng = TransientRenderer()
     eng.wave_paths = wave_paths
     eng.profile = profile
+    eng.binary_bvh = binary_bvh
     st = eng.render(None, w, integ, cam, film, sampler, seed=scene.render.seed if seed is None else seed)
     return film.read(), st, film

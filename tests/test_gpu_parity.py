@@ -281,3 +281,14 @@ def test_full_size_workload_rows_match_oracle(R, oracle):
     per_pix = np.abs(part - ref).sum(axis=(2, 3)) / np.maximum(np.abs(ref).sum(axis=(2, 3)), 1e-4)
     assert (per_pix < 1e-4).mean() >= 0.95
     assert abs(part.sum() / ref.sum() - 1) < 5e-3
+
+
+def test_binary_and_wide_bvh_render_the_same(R):
+    """Both traversal kernels (4-wide default, binary on request) find the same closest hits, so
+    the same random streams give the same film up to float-atomic summation order."""
+    s = scenes.icosphere_box(level=4, width=12, height=12, time_bins=256, sqrt_spp=3, n_incoming=2, max_bounces=6,
+                             seed=21)
+    a, sa, _ = R.render_scene(s)
+    b, sb, _ = R.render_scene(s, binary_bvh=True)
+    assert sa["extension_rays"] == sb["extension_rays"] and sa["contributions"] == sb["contributions"]
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-7)
