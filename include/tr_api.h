@@ -142,7 +142,8 @@ typedef struct {
     float position[3];
     float focus[3];
     float up[3];
-    float fov_deg;        /* -camera-fov: view-plane distance 1/tan(fov/2) (bunnykiller.cpp:1156-1160) */
+    float fov_deg;         /* -camera-fov: view-plane distance 1/tan(fov/2) (bunnykiller.cpp:1156-1160) */
+    float view_plane_dist; /* PinholeCamera's fd (PinholeCamera.h:63-80); used instead of fov_deg if > 0 */
 } tr_camera;
 
 typedef struct {
@@ -157,11 +158,16 @@ typedef struct {
 
 enum {
     TR_RENDER_PROFILE = 1u,     /* time each kernel with HIP events (tr_render_stats.kernel_ms) */
-    TR_RENDER_BINARY_BVH = 2u   /* traverse the binary BVH instead of the default 4-wide one */
+    TR_RENDER_BINARY_BVH = 2u,  /* traverse the binary BVH instead of the default 4-wide one */
+    /* sample Lambertian directions in an orthonormal frame around the normal instead of the
+     * reference's frame (Intersection.h:167-178 builds a tangent that is not orthogonal to n
+     * when n.z != 0); guided renders always use the orthonormal frame */
+    TR_RENDER_ORTHONORMAL_FRAME = 4u
 };
 
 enum {
     TR_K_CAMERA = 0, TR_K_EXTEND = 1, TR_K_SHADE = 2, TR_K_SHADOW = 3, TR_K_FILM_MERGE = 4, TR_K_FILM = 5,
+    TR_K_GUIDE = 6, /* guided renders: training-record bookkeeping and Adam steps */
     TR_K_COUNT = 8
 };
 
@@ -181,6 +187,58 @@ typedef struct {
  * per path and per path for light sampling, all seeded from desc->seed. */
 int tr_render_transient(tr_scene scene, const tr_camera* camera, tr_film film,
                         const tr_render_desc* desc, tr_render_stats* stats);
+
+/* ---- path guiding (no counterpart in the reference) ---------------------------------------
+ * A 48 -> 64 -> 64 -> 64 -> 64 -> 40 ReLU network (BF16 weights, FP32 accumulation on the matrix
+ * cores) maps a path vertex (position, incoming direction, normal, time, medium flag) to a mixture
+ * of 8 von Mises-Fisher lobes. Guided renders draw each next direction from the one-sample MIS
+ * mixture bsdf_fraction * BSDF/phase + (1 - bsdf_fraction) * network, so the film stays an unbiased
+ * estimate, and train the network online (Adam) on the radiance found along sampled directions.
+ * A guide keeps its weights across renders. */
+typedef struct tr_guide_s* tr_guide;
+
+typedef struct {
+    float learning_rate;     /* Adam step size (default 1e-3) */
+    float bsdf_fraction;     /* probability of sampling the BSDF / phase function, in (0, 1] (0.5) */
+    uint32_t train_paths;    /* paths per wavefront whose vertices become training records (65536) */
+    uint32_t batch_size;     /* records per Adam step, a multiple of 64, at most 2^20 (16384) */
+    uint32_t steps_per_wave; /* Adam steps after each wavefront (1) */
+    uint32_t pad0;
+    uint64_t init_seed;      /* deterministic weight initialisation (1) */
+} tr_guide_desc;
+
+typedef struct {
+    uint64_t steps;          /* Adam steps taken */
+    uint64_t records;        /* records passed to tr_guide_train_batch */
+    double last_loss;        /* mean loss of the last step */
+    uint32_t n_params;       /* FP32 master parameters (18,216) */
+    uint32_t flop_per_eval;  /* network FLOP per evaluated vertex (35,840) */
+} tr_guide_info;
+
+/* Creates a guide (desc may be null for the defaults). */
+int tr_guide_create(tr_guide* out, const tr_guide_desc* desc);
+int tr_guide_destroy(tr_guide guide);
+/* Master FP32 parameters, per layer W[in][out] (in-major) then b[out]; setting them resets Adam. */
+int tr_guide_get_params(tr_guide guide, float* params);
+int tr_guide_set_params(tr_guide guide, const float* params);
+/* Network outputs for n raw vertex inputs (12 floats each: position normalised to the scene box,
+ * incoming direction wo, normal (zero in a medium), time / film length, medium flag, 0); out holds
+ * n x 40 floats (lobe i: direction xyz, concentration and weight logits at 5i..5i+4). */
+int tr_guide_eval(tr_guide guide, const float* raw, uint32_t n, float* out);
+/* Same on device pointers; kernel_ms (optional) receives the kernel time from HIP events. */
+int tr_guide_eval_device(tr_guide guide, const float* d_raw, uint32_t n, float* d_out, float* kernel_ms);
+/* Mean loss gradient over n <= batch_size training records (16 floats each: position xyz, t,
+ * wo xyz, medium, normal xyz, weight, sampled direction xyz, 0); weights are normalised by their
+ * mean as in training. grad receives n_params floats. */
+int tr_guide_gradient(tr_guide guide, const float* records, uint32_t n, float* grad, float* loss);
+/* One Adam step on n <= batch_size records. */
+int tr_guide_train_batch(tr_guide guide, const float* records, uint32_t n, float* loss);
+int tr_guide_info_get(tr_guide guide, tr_guide_info* info);
+
+/* tr_render_transient with guided direction sampling (orthonormal shading frame) and online
+ * training of `guide` after every wavefront. */
+int tr_render_transient_guided(tr_scene scene, const tr_camera* camera, tr_film film,
+                               const tr_render_desc* desc, tr_guide guide, tr_render_stats* stats);
 
 /* ---- misc --------------------------------------------------------------------------------- */
 

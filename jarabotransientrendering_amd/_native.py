@@ -20,9 +20,15 @@ TR_ERR_STATE = -4
 
 TR_RENDER_PROFILE = 1
 TR_RENDER_BINARY_BVH = 2
-K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FILM_MERGE, K_FILM = 0, 1, 2, 3, 4, 5
+TR_RENDER_ORTHONORMAL_FRAME = 4
+K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FILM_MERGE, K_FILM, K_GUIDE = 0, 1, 2, 3, 4, 5, 6
 K_NAMES = {K_CAMERA: "camera", K_EXTEND: "extend", K_SHADE: "shade", K_SHADOW: "shadow",
-           K_FILM_MERGE: "film_merge", K_FILM: "film"}
+           K_FILM_MERGE: "film_merge", K_FILM: "film", K_GUIDE: "guide"}
+# guiding network (tr_guide_*): parameter count, raw input / output / record widths
+GUIDE_PARAMS = 18216
+GUIDE_RAW = 12
+GUIDE_OUT = 40
+GUIDE_RECORD = 16
 TR_K_COUNT = 8
 
 
@@ -59,7 +65,7 @@ class FilmDesc(ctypes.Structure):
 
 class CameraDesc(ctypes.Structure):
     _fields_ = [("position", ctypes.c_float * 3), ("focus", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
-                ("fov_deg", ctypes.c_float)]
+                ("fov_deg", ctypes.c_float), ("view_plane_dist", ctypes.c_float)]
 
 
 class RenderDesc(ctypes.Structure):
@@ -81,6 +87,17 @@ class RenderStats(ctypes.Structure):
         d["kernel_ms"] = {K_NAMES[k]: float(self.kernel_ms[k]) for k in K_NAMES}
         d["kernel_launches"] = {K_NAMES[k]: int(self.kernel_launches[k]) for k in K_NAMES}
         return d
+
+
+class GuideDesc(ctypes.Structure):
+    _fields_ = [("learning_rate", ctypes.c_float), ("bsdf_fraction", ctypes.c_float),
+                ("train_paths", ctypes.c_uint32), ("batch_size", ctypes.c_uint32),
+                ("steps_per_wave", ctypes.c_uint32), ("pad0", ctypes.c_uint32), ("init_seed", ctypes.c_uint64)]
+
+
+class GuideInfo(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("records", ctypes.c_uint64), ("last_loss", ctypes.c_double),
+                ("n_params", ctypes.c_uint32), ("flop_per_eval", ctypes.c_uint32)]
 
 
 # exported symbol -> (restype, argtypes); the header declares exactly these
@@ -107,6 +124,21 @@ SIGNATURES = {
                                               ctypes.POINTER(ctypes.c_void_p)]),
     "tr_render_transient": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CameraDesc), ctypes.c_void_p,
                                            ctypes.POINTER(RenderDesc), ctypes.POINTER(RenderStats)]),
+    "tr_guide_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(GuideDesc)]),
+    "tr_guide_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "tr_guide_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "tr_guide_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "tr_guide_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "tr_guide_eval_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                            ctypes.POINTER(ctypes.c_float)]),
+    "tr_guide_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                         ctypes.POINTER(ctypes.c_float)]),
+    "tr_guide_train_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                            ctypes.POINTER(ctypes.c_float)]),
+    "tr_guide_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GuideInfo)]),
+    "tr_render_transient_guided": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CameraDesc), ctypes.c_void_p,
+                                                  ctypes.POINTER(RenderDesc), ctypes.c_void_p,
+                                                  ctypes.POINTER(RenderStats)]),
     "tr_last_error": (ctypes.c_char_p, []),
     "tr_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                       ctypes.POINTER(ctypes.c_uint64)]),

@@ -11,8 +11,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(LIBDIR, "libtransient_hip.so")
 
-SOURCES = ["tr_kernels.hip", "tr_bvh.cpp"]
-HEADERS = ["tr_device.h", "tr_internal.h"]
+SOURCES = ["tr_kernels.hip", "tr_guide.hip", "tr_bvh.cpp"]
+HEADERS = ["tr_device.h", "tr_internal.h", "tr_guide.h", "tr_guide_host.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -23,17 +23,28 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
+CLI_SRC = os.path.join(PKG, "cli", "render_scene.cpp")
+CLI = os.path.join(LIBDIR, "render_scene")
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr[-4000:]}")
+        raise RuntimeError(f"{cmd[0]} failed ({res.returncode}):\n{res.stderr[-4000:]}")
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+          *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp], verbose)
     os.replace(tmp, LIB)
+    # the C++ host driver over the C ABI (plain g++; finds the library next to itself)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), CLI_SRC,
+          "-L", LIBDIR, "-ltransient_hip", "-Wl,-rpath,$ORIGIN", "-o", CLI + ".tmp"], verbose)
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

@@ -69,3 +69,10 @@ struct alignas(16) LightHost {
     float I[4];       // RGB intensity (rect: scalar replicated), w = type (as float)
 };
 static_assert(sizeof(LightHost) == 80, "light layout");
+
+// ---- shared host helpers (defined in tr_kernels.hip) ----------------------------------------
+namespace tri {
+int fail(int code, const std::string& msg);  // records the message for tr_last_error, returns code
+int check_device();                          // TR_OK if the current device is a gfx950
+int compute_units();                         // compute units of the current device
+}  // namespace tri

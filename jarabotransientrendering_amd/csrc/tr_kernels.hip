@@ -33,6 +33,8 @@
 
 #include "tr_api.h"
 #include "tr_device.h"
+#include "tr_guide.h"
+#include "tr_guide_host.h"
 #include "tr_internal.h"
 
 using namespace trd;
@@ -105,6 +107,7 @@ struct WaveDev {
     float4* sh_d;     // direction xyz, w = film element index (uint bits)
     float4* sh_c;     // contribution rgb, w = film pixel (uint bits)
     uint8_t* sh_vis;  // 1 if the shadow ray is unoccluded
+    uint32_t* sh_tag; // guided renders: training record of the contribution (or ~0)
     uint32_t* counters;
 };
 
@@ -219,6 +222,24 @@ __device__ __forceinline__ f3 sample_lambert(f3 n, float e1, float e2, float* co
     return normalize(tu * l.x + n * l.y + tv * l.z);
 }
 
+// Cosine-weighted direction in an orthonormal frame around n (Duff et al. 2017); the density is
+// exactly cos/pi, which one-sample MIS with the guiding mixture needs.
+__device__ __forceinline__ f3 sample_lambert_onb(f3 n, float e1, float e2, float* cos_sampled)
+{
+    float phi = 2.0f * kPi * e1;
+    float ct = sqrtf(e2);
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    float sign = copysignf(1.0f, n.z);
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    f3 t = mk(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    f3 bt = mk(b, sign + n.y * n.y * a, -n.y);
+    *cos_sampled = ct;
+    return normalize(t * (cp * st) + bt * (sp * st) + n * ct);
+}
+
 // Isotropic phase-function direction: uniform sphere rotated so that +y maps onto wi
 // (Sampling.h:80-94, IsotropicPhaseFunction.h:54-60, VectorN.h:429-523).
 __device__ __forceinline__ f3 sample_isotropic(f3 wi, float e1, float e2)
@@ -267,9 +288,29 @@ __device__ __forceinline__ f3 sample_isotropic(f3 wi, float e1, float e2)
 
 // ---------------------------------------------------------------------------------------------
 // shade: one thread per queued path whose ray has just been traced
-__global__ void __launch_bounds__(kShadeBlock) shade_kernel(SceneDev S, RenderDev R, WaveDev Wv, int cur)
+// Shading modes: the reference's shading frame (default), an orthonormal frame (same estimator
+// without the frame quirk), and guided sampling (orthonormal frame + one-sample MIS of the BSDF /
+// phase function with the network's vMF mixture).
+enum { kShadeRef = 0, kShadeOnb = 1, kShadeGuided = 2 };
+constexpr int kShadeBlockGuided = 256;  // 4 wavefronts x 10 KiB of LDS for the network
+
+__device__ __forceinline__ float mean3(f3 v) { return (v.x + v.y + v.z) * (1.0f / 3.0f); }
+
+template <int MODE>
+__global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kShadeBlock)
+    shade_kernel(SceneDev S, RenderDev R, WaveDev Wv, int cur, trg::GuideView G)
 {
+    constexpr bool GUIDED = MODE == kShadeGuided;
     __shared__ uint32_t scan[kShadeBlock / 64 + 1];
+    __shared__ float glds[GUIDED ? (kShadeBlockGuided / 64) * 64 * trg::kRow : 1];
+    __shared__ uint4 gfrag[GUIDED ? trg::kFwdFrags * 64 : 1];
+    __shared__ float gbias[GUIDED ? trg::kBiasPad : 1];
+    if constexpr (GUIDED) {
+        // the network's weights stay in LDS for the whole block (38 KiB + biases)
+        for (uint32_t k = threadIdx.x; k < uint32_t(trg::kFwdFrags * 64); k += blockDim.x) gfrag[k] = G.frag[k];
+        for (uint32_t k = threadIdx.x; k < uint32_t(trg::kBiasPad); k += blockDim.x) gbias[k] = G.bias[k];
+        __syncthreads();
+    }
     const uint32_t count = Wv.counters[cur];
     if (blockIdx.x == 0 && threadIdx.x == 0) Wv.counters[C_FETCH_EXT] = 0;  // for the next extend
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -277,21 +318,33 @@ __global__ void __launch_bounds__(kShadeBlock) shade_kernel(SceneDev S, RenderDe
         const uint32_t i = base + threadIdx.x;
         bool live = i < count;
         uint32_t slot = live ? Wv.q[cur][i] : 0u;
-        bool emit_shadow = false, cont = false;
+        bool emit_shadow = false, cont = false, want = false;
         float4 sh_o = {}, sh_d = {}, sh_c = {};
         float4 n_o = {}, n_d = {}, n_thr = {};
         uint4 n_misc = {}, n_extra = {};
+        uint32_t sh_tag = 0xFFFFFFFFu, tag = 0xFFFFFFFFu;
+        // state of a vertex that goes on to sample a direction
+        f3 f = {}, pos = {}, n = {}, d = {};
+        bool medium_event = false;
+        int32_t mat = -1;
+        float pi = 1.0f, time = 0.0f, nsampled = 0.0f;
+        uint64_t pst = 0, nst = 0, pinc = 0;
+        uint4 misc = {}, extra = {};
         if (live) {
             float4 o4 = Wv.ray_o[slot], d4 = Wv.ray_d[slot], thr = Wv.thr[slot];
-            uint4 misc = Wv.misc[slot], extra = Wv.extra[slot];
+            misc = Wv.misc[slot];
+            extra = Wv.extra[slot];
             float2 h = Wv.hit[slot];
             int32_t tri = __float_as_int(h.y);
             uint32_t level = extra.z & 0xFFu;
-            f3 o = mk(o4), d = mk(d4);
+            f3 o = mk(o4);
+            d = mk(d4);
             float sampled = o4.w;
             if (level == 0 && R.unwarp) extra.w = __float_as_uint(tri >= 0 ? h.x : 3.402823466e+38f);
+            if constexpr (GUIDED) {
+                if (slot % G.stride == 0u && slot / G.stride < G.train_paths) tag = (slot / G.stride) * G.max_levels;
+            }
             // correct_next_event_distance (BidirectionalPathTracing.h:275-307)
-            bool medium_event = false;
             float dist, pd = 1.0f;
             bool alive = true;
             if (!S.has_medium) {
@@ -306,23 +359,23 @@ __global__ void __launch_bounds__(kShadeBlock) shade_kernel(SceneDev S, RenderDe
                 pd = expf(-dist * S.lambda) * S.lambda;
             }
             if (alive) {
-                f3 f = mk(thr);
+                f = mk(thr);
                 if (S.has_medium) {
                     f = mk(f.x * expf(-dist * S.sigma_t.x), f.y * expf(-dist * S.sigma_t.y), f.z * expf(-dist * S.sigma_t.z));
                     f = f * (1.0f / pd);
                 }
-                float time = thr.w + dist;
-                f3 pos = o + d * dist;
-                f3 n = mk(0.0f, 0.0f, 0.0f);
-                int32_t mat = -1;
+                time = thr.w + dist;
+                pos = o + d * dist;
+                n = mk(0.0f, 0.0f, 0.0f);
                 if (!medium_event) {
                     float4 nn = S.tri_normal[tri];
                     n = mk(nn);
                     mat = __float_as_int(nn.w);
                 }
-                uint64_t nst = u64(extra.x, extra.y);
+                nst = u64(extra.x, extra.y);
                 uint64_t pid = u64(misc.x, misc.y);
-                uint64_t ninc = pcg_inc(2 * pid + 1), pinc = pcg_inc(2 * pid);
+                uint64_t ninc = pcg_inc(2 * pid + 1);
+                pinc = pcg_inc(2 * pid);
                 // ---- next-event estimation at this vertex (connect_paths :976-1045,
                 //      connect_vertex_with_light :650-692)
                 if (int(level) + 1 <= R.max_bounces && time <= R.total_time) {
@@ -382,47 +435,125 @@ __global__ void __launch_bounds__(kShadeBlock) shade_kernel(SceneDev S, RenderDe
                             sh_o = make_float4(lp.x, lp.y, lp.z, ldist);
                             sh_d = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(fidx));
                             sh_c = make_float4(val.x, val.y, val.z, __uint_as_float(extra.z >> 8));
+                            if (GUIDED && tag != 0xFFFFFFFFu) sh_tag = tag + level;
                         }
                     }
                 }
                 // ---- termination: later vertices cannot contribute past the bounce cap or the
                 //      film window; otherwise Russian roulette on the mean albedo (:447-481)
                 if (int(level) + 1 < R.max_bounces && time <= R.total_time) {
-                    uint64_t pst = u64(misc.z, misc.w);
+                    pst = u64(misc.z, misc.w);
                     float albedo = medium_event ? S.medium_albedo : S.mats[mat].w;
                     float eps = pcg_float(pst, pinc);
                     if (!(eps > albedo)) {
-                        float pi = albedo;
-                        float nsampled = __int_as_float(0x7f800000);
+                        pi = albedo;
+                        nsampled = __int_as_float(0x7f800000);
                         if (S.has_medium) {
                             float e = pcg_float(pst, pinc);
                             nsampled = -logf(e) / S.lambda;
                         }
-                        f3 nd, fa;
-                        float pa;
-                        if (medium_event) {
-                            float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
-                            nd = sample_isotropic(d, e1, e2);
-                            fa = mk(S.sigma_s.x, S.sigma_s.y, S.sigma_s.z) * kInv4Pi;
-                            pa = kInv4Pi;
-                        } else {
-                            (void)pcg_float(pst, pinc);  // BSDF pick of a single-BSDF UberMaterial
-                            float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
-                            float ct;
-                            nd = sample_lambert(n, e1, e2, &ct);
-                            float4 kd = S.mats[mat];
-                            fa = mk(kd.x, kd.y, kd.z) * (kInvPi * fabsf(dot(n, nd)));
-                            pa = ct * kInvPi;
-                        }
-                        f3 nf = mul(f, fa) * (1.0f / (pi * pa));
-                        cont = true;
-                        n_o = make_float4(pos.x, pos.y, pos.z, nsampled);
-                        n_d = make_float4(nd.x, nd.y, nd.z, 0.0f);
-                        n_thr = make_float4(nf.x, nf.y, nf.z, time);
-                        n_misc = make_uint4(misc.x, misc.y, uint32_t(pst), uint32_t(pst >> 32));
-                        n_extra = make_uint4(uint32_t(nst), uint32_t(nst >> 32), extra.z + 1u, extra.w);
+                        want = true;
+                        if (GUIDED && tag != 0xFFFFFFFFu) tag += level;
                     }
                 }
+            }
+        }
+        if (!want) tag = 0xFFFFFFFFu;
+        // ---- guided: the network's mixture for every vertex of this wavefront that samples a
+        //      direction (all 64 lanes take part in the matrix-core evaluation)
+        float raw[GUIDED ? trg::kRaw : 1];
+        float* mine = glds + (GUIDED ? (threadIdx.x >> 6) * 64 * trg::kRow : 0);
+        if constexpr (GUIDED) {
+            const f3 pn = mk((pos.x - G.bb_lo.x) * G.bb_inv.x, (pos.y - G.bb_lo.y) * G.bb_inv.y,
+                             (pos.z - G.bb_lo.z) * G.bb_inv.z);
+            raw[0] = pn.x, raw[1] = pn.y, raw[2] = pn.z;
+            raw[3] = -d.x, raw[4] = -d.y, raw[5] = -d.z;
+            raw[6] = n.x, raw[7] = n.y, raw[8] = n.z;
+            raw[9] = fminf(fmaxf(time / R.total_time, 0.0f), 1.0f);
+            raw[10] = medium_event ? 1.0f : 0.0f;
+            raw[11] = 0.0f;
+            if (__ballot(want) != 0ull) {
+                const int lane = int(__lane_id());
+                float feat[trg::kFeat];
+                trg::encode_features(raw, feat);
+                trg::store_feature_row(reinterpret_cast<uint32_t*>(mine) + lane * trg::kRow, feat);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                trg::mlp_forward_wave<1>(mine, gfrag, gbias, nullptr);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+        if (want) {
+            f3 nd, fa;
+            float pa;
+            bool ok = true;
+            if constexpr (!GUIDED) {
+                if (medium_event) {
+                    float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
+                    nd = sample_isotropic(d, e1, e2);
+                    fa = mk(S.sigma_s.x, S.sigma_s.y, S.sigma_s.z) * kInv4Pi;
+                    pa = kInv4Pi;
+                } else {
+                    (void)pcg_float(pst, pinc);  // BSDF pick of a single-BSDF UberMaterial
+                    float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
+                    float ct;
+                    nd = MODE == kShadeRef ? sample_lambert(n, e1, e2, &ct) : sample_lambert_onb(n, e1, e2, &ct);
+                    float4 kd = S.mats[mat];
+                    fa = mk(kd.x, kd.y, kd.z) * (kInvPi * fabsf(dot(n, nd)));
+                    pa = ct * kInvPi;
+                }
+            } else {
+                const float alpha = G.alpha;
+                float usel = pcg_float(pst, pinc);
+                float e1, e2;
+                if (usel < alpha) {
+                    e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
+                    float ct;
+                    nd = medium_event ? sample_isotropic(d, e1, e2) : sample_lambert_onb(n, e1, e2, &ct);
+                }
+                const trg::MixtureRef mix = trg::mixture_ref(mine + __lane_id() * trg::kRow);
+                if (!(usel < alpha)) {
+                    float ul = pcg_float(pst, pinc);
+                    e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
+                    trg::Lobe L = trg::lobe_at(mix, trg::pick_lobe(mix, ul));
+                    trg::vmf_sample(L, e1, e2, &nd.x, &nd.y, &nd.z);
+                }
+                float q = trg::mixture_pdf(mix, nd.x, nd.y, nd.z);
+                if (medium_event) {
+                    fa = mk(S.sigma_s.x, S.sigma_s.y, S.sigma_s.z) * kInv4Pi;
+                    pa = alpha * kInv4Pi + (1.0f - alpha) * q;
+                } else {
+                    float c = dot(n, nd);
+                    ok = c > 0.0f;  // below the surface the one-sided Lambertian reflects nothing
+                    float4 kd = S.mats[mat];
+                    fa = mk(kd.x, kd.y, kd.z) * (kInvPi * fmaxf(c, 0.0f));
+                    pa = alpha * fmaxf(c, 0.0f) * kInvPi + (1.0f - alpha) * q;
+                }
+                ok = ok && pa > 0.0f;
+                if (ok && tag != 0xFFFFFFFFu) {
+                    // training record: inputs, sampled direction and pi / throughput (the later
+                    // contributions along this direction, divided by the throughput after this
+                    // vertex, times BSDF x cosine / density)
+                    float mf = mean3(f);
+                    float4* rec = G.rec + size_t(tag) * 4;
+                    rec[0] = make_float4(raw[0], raw[1], raw[2], raw[9]);
+                    rec[1] = make_float4(raw[3], raw[4], raw[5], raw[10]);
+                    rec[2] = make_float4(raw[6], raw[7], raw[8], mf > 0.0f ? pi / mf : 0.0f);
+                    rec[3] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+                    G.n_rec[tag / G.max_levels] = tag % G.max_levels + 1u;
+                }
+            }
+            if (ok) {
+                f3 nf = mul(f, fa) * (1.0f / (pi * pa));
+                cont = true;
+                n_o = make_float4(pos.x, pos.y, pos.z, nsampled);
+                n_d = make_float4(nd.x, nd.y, nd.z, 0.0f);
+                n_thr = make_float4(nf.x, nf.y, nf.z, time);
+                n_misc = make_uint4(misc.x, misc.y, uint32_t(pst), uint32_t(pst >> 32));
+                n_extra = make_uint4(uint32_t(nst), uint32_t(nst >> 32), extra.z + 1u, extra.w);
             }
         }
         uint32_t sh_pos = block_append(Wv.counters + C_SHADOW, emit_shadow ? 1u : 0u, scan);
@@ -430,6 +561,7 @@ __global__ void __launch_bounds__(kShadeBlock) shade_kernel(SceneDev S, RenderDe
             Wv.sh_o[sh_pos] = sh_o;
             Wv.sh_d[sh_pos] = sh_d;
             Wv.sh_c[sh_pos] = sh_c;
+            if constexpr (GUIDED) Wv.sh_tag[sh_pos] = sh_tag;
         }
         uint32_t q_pos = block_append(Wv.counters + (cur ^ 1), cont ? 1u : 0u, scan);
         if (cont) {
@@ -491,7 +623,8 @@ constexpr uint32_t kFilmRange = 4096;
 constexpr uint32_t kFilmLdsBytes = 48 * 1024;
 constexpr uint32_t kFilmMaxSlots = 8;
 
-__global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev Wv, uint32_t pix_slots)
+template <bool GUIDED>
+__global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev Wv, uint32_t pix_slots, float* contrib)
 {
     extern __shared__ float hist[];          // pix_slots * T * 3
     __shared__ uint32_t slot_pix[kFilmMaxSlots];
@@ -505,6 +638,11 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
         for (uint32_t r = start + threadIdx.x; r < end; r += blockDim.x) {
             if (!Wv.sh_vis[r]) continue;
             float4 c = Wv.sh_c[r];
+            if constexpr (GUIDED) {
+                // luminance of the contribution, credited to the training record of its vertex
+                uint32_t tag = Wv.sh_tag[r];
+                if (tag != 0xFFFFFFFFu) atomicAdd(contrib + tag, (c.x + c.y + c.z) * (1.0f / 3.0f));
+            }
             uint32_t pix = __float_as_uint(c.w);
             uint32_t bin = __float_as_uint(Wv.sh_d[r].w) - pix * T;
             int slot = -1;
@@ -671,13 +809,20 @@ struct WaveBuffers {
     uint32_t *q0 = nullptr, *q1 = nullptr;
     float4 *sh_o = nullptr, *sh_d = nullptr, *sh_c = nullptr;
     uint8_t* sh_vis = nullptr;
+    uint32_t* sh_tag = nullptr;  // allocated by the first guided render
     uint32_t* counters = nullptr;
 
     void release()
     {
         dev_free(ray_o); dev_free(ray_d); dev_free(thr); dev_free(misc); dev_free(extra); dev_free(hit);
         dev_free(q0); dev_free(q1); dev_free(sh_o); dev_free(sh_d); dev_free(sh_c); dev_free(sh_vis); dev_free(counters);
+        dev_free(sh_tag);
         cap = 0;
+    }
+    int ensure_tags()
+    {
+        if (sh_tag || cap == 0) return TR_OK;
+        return dev_alloc(&sh_tag, cap);
     }
     int ensure(uint32_t n)
     {
@@ -699,12 +844,25 @@ struct WaveBuffers {
     {
         WaveDev w;
         w.ray_o = ray_o; w.ray_d = ray_d; w.thr = thr; w.misc = misc; w.extra = extra; w.hit = hit;
-        w.q[0] = q0; w.q[1] = q1; w.sh_o = sh_o; w.sh_d = sh_d; w.sh_c = sh_c; w.sh_vis = sh_vis; w.counters = counters;
+        w.q[0] = q0; w.q[1] = q1; w.sh_o = sh_o; w.sh_d = sh_d; w.sh_c = sh_c; w.sh_vis = sh_vis; w.sh_tag = sh_tag;
+        w.counters = counters;
         return w;
     }
 };
 
 }  // namespace
+
+namespace tri {
+int fail(int code, const std::string& msg) { return ::fail(code, msg); }
+int check_device() { return ::check_device(); }
+int compute_units()
+{
+    int dev = 0, cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0) return 256;
+    return cu;
+}
+}  // namespace tri
 
 struct tr_scene_s {
     std::vector<float> tris;        // n*9
@@ -723,6 +881,7 @@ struct tr_scene_s {
     uint32_t n_nodes = 0, n_nodes4 = 0;
     int depth = 0, depth4 = 0, stack4 = 0;
     bool wide = true;   // traverse the 4-wide BVH (tr_render_desc flags can select the binary one)
+    float bb_lo[3] = {0, 0, 0}, bb_hi[3] = {1, 1, 1};  // bounds of the triangles (set at commit)
     double build_ms = 0.0;
     uint64_t dev_bytes = 0;
     WaveBuffers wave;
@@ -938,6 +1097,15 @@ int tr_scene_commit(tr_scene s)
         std::memcpy(&w, &mi, 4);
         normals[i] = make_float4(c[0] * inv, c[1] * inv, c[2] * inv, w);
     }
+    for (int k = 0; k < 3; ++k) {
+        s->bb_lo[k] = n ? 3.4e38f : 0.0f;
+        s->bb_hi[k] = n ? -3.4e38f : 1.0f;
+    }
+    for (size_t v = 0; v < size_t(n) * 3; ++v)
+        for (int k = 0; k < 3; ++k) {
+            s->bb_lo[k] = std::min(s->bb_lo[k], s->tris[3 * v + size_t(k)]);
+            s->bb_hi[k] = std::max(s->bb_hi[k], s->tris[3 * v + size_t(k)]);
+        }
     std::vector<uint2> ids(n > 0 ? n : 1);
     for (uint32_t i = 0; i < n; ++i) ids[i] = make_uint2(s->tri_geom[i], s->tri_prim[i]);
     size_t nn = b.nodes.size(), nt = b.tris.size(), nm = std::max<size_t>(1, s->mats.size() / 4),
@@ -1141,7 +1309,12 @@ int tr_film_device_buffers(tr_film f, float** d_sum, float** d_comp)
     return TR_OK;
 }
 
-int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* d, tr_render_stats* stats)
+}  // extern "C"
+
+namespace {
+
+int render_impl(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* d, tr_guide guide,
+                tr_render_stats* stats)
 {
     auto t_start = std::chrono::steady_clock::now();
     if (!s || !cam || !f || !d) return fail(TR_ERR_INVALID, "tr_render_transient: null argument");
@@ -1163,6 +1336,14 @@ int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_re
     wave = uint32_t(std::min<uint64_t>(wave, std::max<uint64_t>(total, 1)));
     int rc = s->wave.ensure(wave);
     if (rc) return rc;
+    trg::GuideView GV{};
+    if (guide) {
+        if (d->max_bounces < 1) return fail(TR_ERR_INVALID, "tr_render_transient_guided: max_bounces must be >= 1");
+        if ((rc = s->wave.ensure_tags())) return rc;
+        if ((rc = trg::guide_begin_render(guide, wave, uint32_t(d->max_bounces), s->bb_lo, s->bb_hi, s->stream, &GV)))
+            return rc;
+    }
+    const int mode = guide ? kShadeGuided : ((d->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef);
 
     // camera set-up (PinholeCamera.h:56-80; bunnykiller.cpp:1156-1160)
     auto nrm = [](float* v) {
@@ -1204,7 +1385,7 @@ int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_re
     R.nv = make_float3(nv[0], nv[1], nv[2]);
     R.uv = make_float3(uv[0], uv[1], uv[2]);
     R.vv = make_float3(vv[0], vv[1], vv[2]);
-    R.fd = float(1. / std::tan(double(fov)));
+    R.fd = cam->view_plane_dist > 0.f ? cam->view_plane_dist : float(1. / std::tan(double(fov)));
     R.xw2 = float(double(fd.width) / 2.0);
     R.yw2 = float(double(fd.height) / 2.0);
     R.ratio = float(fd.width) / float(fd.height);
@@ -1264,27 +1445,45 @@ int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_re
     for (uint64_t base = 0; base < total; base += wave) {
         uint32_t n = uint32_t(std::min<uint64_t>(wave, total - base));
         ++n_waves;
+        if (guide) {
+            if ((rc = begin(TR_K_GUIDE))) return rc;
+            if ((rc = trg::guide_wave_begin(guide, s->stream))) return rc;
+            if ((rc = end())) return rc;
+        }
         if ((rc = begin(TR_K_CAMERA))) return rc;
         camera_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(S, R, Wv, uint32_t(base), n);
         if ((rc = end())) return rc;
         uint32_t tg = persistent_grid(n, s->compute_units);
         uint32_t fg = std::max(1u, std::min((n + kFilmRange - 1) / kFilmRange, uint32_t(std::max(s->compute_units, 1)) * 4u));
-        uint32_t sg = std::max(1u, std::min((n + kShadeBlock - 1) / kShadeBlock, uint32_t(std::max(s->compute_units, 1)) * 2u));
+        const uint32_t sblock = mode == kShadeGuided ? kShadeBlockGuided : kShadeBlock;
+        const uint32_t sgrid_cap = uint32_t(std::max(s->compute_units, 1)) * (mode == kShadeGuided ? 8u : 2u);
+        uint32_t sg = std::max(1u, std::min((n + sblock - 1) / sblock, sgrid_cap));
         int cur = 0;
         for (int b = 0; b < bounces; ++b) {
             if ((rc = begin(TR_K_EXTEND))) return rc;
             TR_STACK_DISPATCH(wide, s, extend_kernel, tg, kTraceBlock, s->stream, S, Wv, cur);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADE))) return rc;
-            shade_kernel<<<sg, kShadeBlock, 0, s->stream>>>(S, R, Wv, cur);
+            if (mode == kShadeGuided)
+                shade_kernel<kShadeGuided><<<sg, kShadeBlockGuided, 0, s->stream>>>(S, R, Wv, cur, GV);
+            else if (mode == kShadeOnb)
+                shade_kernel<kShadeOnb><<<sg, kShadeBlock, 0, s->stream>>>(S, R, Wv, cur, GV);
+            else
+                shade_kernel<kShadeRef><<<sg, kShadeBlock, 0, s->stream>>>(S, R, Wv, cur, GV);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADOW))) return rc;
             TR_STACK_DISPATCH(wide, s, shadow_kernel, tg, kTraceBlock, s->stream, S, Wv);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_FILM))) return rc;
-            film_kernel<<<fg, kFilmBlock, film_lds, s->stream>>>(R, Wv, pix_slots);
+            if (guide) film_kernel<true><<<fg, kFilmBlock, film_lds, s->stream>>>(R, Wv, pix_slots, GV.contrib);
+            else film_kernel<false><<<fg, kFilmBlock, film_lds, s->stream>>>(R, Wv, pix_slots, nullptr);
             if ((rc = end())) return rc;
             cur ^= 1;
+        }
+        if (guide) {
+            if ((rc = begin(TR_K_GUIDE))) return rc;
+            if ((rc = trg::guide_wave_end(guide, s->stream))) return rc;
+            if ((rc = end())) return rc;
         }
     }
     {
@@ -1317,6 +1516,22 @@ int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_re
         stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     }
     return TR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tr_render_transient(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* d, tr_render_stats* stats)
+{
+    return render_impl(s, cam, f, d, nullptr, stats);
+}
+
+int tr_render_transient_guided(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* d, tr_guide g,
+                               tr_render_stats* stats)
+{
+    if (!g) return fail(TR_ERR_INVALID, "tr_render_transient_guided: null guide");
+    return render_impl(s, cam, f, d, g, stats);
 }
 
 }  // extern "C"

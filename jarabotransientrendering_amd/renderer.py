@@ -311,6 +311,66 @@ class BidirectionalPathTracing:
         pass
 
 
+class PathGuide:
+    """Online-trained vMF-mixture guiding network (tr_guide_*; no counterpart in the reference).
+    Pass it to TransientRenderer.render(..., guide=...) to sample directions from the one-sample
+    MIS mixture of the BSDF/phase function and the network; weights persist across renders."""
+
+    def __init__(self, learning_rate: float = 1e-3, bsdf_fraction: float = 0.5, train_paths: int = 1 << 16,
+                 batch_size: int = 1 << 14, steps_per_wave: int = 1, init_seed: int = 1):
+        self.handle = ctypes.c_void_p()
+        d = nat.GuideDesc(learning_rate, bsdf_fraction, train_paths, batch_size, steps_per_wave, 0, init_seed)
+        nat.check(nat.lib().tr_guide_create(ctypes.byref(self.handle), ctypes.byref(d)))
+        self.batch_size = batch_size
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                nat.lib().tr_guide_destroy(h)
+            except Exception:
+                pass
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.empty(nat.GUIDE_PARAMS, np.float32)
+        nat.check(nat.lib().tr_guide_get_params(self.handle, nat.as_ptr(p)))
+        return p
+
+    @params.setter
+    def params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.shape != (nat.GUIDE_PARAMS,):
+            raise ValueError(f"expected {nat.GUIDE_PARAMS} parameters")
+        nat.check(nat.lib().tr_guide_set_params(self.handle, nat.as_ptr(p)))
+
+    def eval(self, raw) -> np.ndarray:
+        raw = np.ascontiguousarray(raw, dtype=np.float32).reshape(-1, nat.GUIDE_RAW)
+        out = np.empty((len(raw), nat.GUIDE_OUT), np.float32)
+        nat.check(nat.lib().tr_guide_eval(self.handle, nat.as_ptr(raw), len(raw), nat.as_ptr(out)))
+        return out
+
+    def gradient(self, records):
+        rec = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, nat.GUIDE_RECORD)
+        g = np.empty(nat.GUIDE_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        nat.check(nat.lib().tr_guide_gradient(self.handle, nat.as_ptr(rec), len(rec), nat.as_ptr(g),
+                                              ctypes.byref(loss)))
+        return g, float(loss.value)
+
+    def train_batch(self, records) -> float:
+        rec = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, nat.GUIDE_RECORD)
+        loss = ctypes.c_float()
+        nat.check(nat.lib().tr_guide_train_batch(self.handle, nat.as_ptr(rec), len(rec), ctypes.byref(loss)))
+        return float(loss.value)
+
+    def info(self) -> dict:
+        i = nat.GuideInfo()
+        nat.check(nat.lib().tr_guide_info_get(self.handle, ctypes.byref(i)))
+        return {"steps": int(i.steps), "records": int(i.records), "last_loss": float(i.last_loss),
+                "n_params": int(i.n_params), "flop_per_eval": int(i.flop_per_eval)}
+
+
 class TransientRenderer:
     """TransientRenderer::render (src/Renderer/TransientRenderer.h:77-201)."""
 
@@ -320,10 +380,12 @@ class TransientRenderer:
         self.wave_paths = 0
         self.profile = False
         self.binary_bvh = False   # traverse the binary BVH instead of the 4-wide one
+        self.orthonormal_frame = False  # Lambertian sampling frame without the reference's quirk
         self.last_stats: Optional[dict] = None
 
     def render(self, name: Optional[str], world: World, integrator: Optional[BidirectionalPathTracing],
-               camera: PinholeCamera, film: StreakCameraFilm, sampler: StratifiedSampler, seed: int = 0) -> dict:
+               camera: PinholeCamera, film: StreakCameraFilm, sampler: StratifiedSampler, seed: int = 0,
+               guide: Optional[PathGuide] = None) -> dict:
         if integrator is None:
             raise RuntimeError("Error: Need to define an integrator for rendering\n")
         if not world.frozen:
@@ -333,13 +395,18 @@ class TransientRenderer:
         if (sampler.size_x, sampler.size_y) != (film.width, film.height):
             raise ValueError("sampler and film sizes differ")
         integrator.preprocess()
-        flags = (nat.TR_RENDER_PROFILE if self.profile else 0) | (nat.TR_RENDER_BINARY_BVH if self.binary_bvh else 0)
+        flags = ((nat.TR_RENDER_PROFILE if self.profile else 0) | (nat.TR_RENDER_BINARY_BVH if self.binary_bvh else 0)
+                 | (nat.TR_RENDER_ORTHONORMAL_FRAME if self.orthonormal_frame else 0))
         d = nat.RenderDesc(sampler.sqrt_spp, integrator.incoming_samples, integrator.max_path_size, 0, int(seed),
                            int(self.wave_paths), flags)
         st = nat.RenderStats()
         cam = camera.desc()
-        nat.check(nat.lib().tr_render_transient(world.handle, ctypes.byref(cam), film.handle, ctypes.byref(d),
-                                                ctypes.byref(st)))
+        if guide is None:
+            nat.check(nat.lib().tr_render_transient(world.handle, ctypes.byref(cam), film.handle, ctypes.byref(d),
+                                                    ctypes.byref(st)))
+        else:
+            nat.check(nat.lib().tr_render_transient_guided(world.handle, ctypes.byref(cam), film.handle,
+                                                           ctypes.byref(d), guide.handle, ctypes.byref(st)))
         self.last_stats = st.as_dict()
         if name:
             film.write(name)
@@ -381,12 +448,14 @@ def setup_scene(scene):
 
 
 def render_scene(scene, seed: Optional[int] = None, wave_paths: int = 0, profile: bool = False,
-                 binary_bvh: bool = False):
+                 binary_bvh: bool = False, orthonormal_frame: bool = False, guide: Optional[PathGuide] = None):
     """Render a `scenes.Scene`; returns (normalised film [rows][W][T][3], stats, film object)."""
     w, cam, sampler, film, integ = setup_scene(scene)
     eng = TransientRenderer()
     eng.wave_paths = wave_paths
     eng.profile = profile
     eng.binary_bvh = binary_bvh
-    st = eng.render(None, w, integ, cam, film, sampler, seed=scene.render.seed if seed is None else seed)
+    eng.orthonormal_frame = orthonormal_frame
+    st = eng.render(None, w, integ, cam, film, sampler, seed=scene.render.seed if seed is None else seed,
+                    guide=guide)
     return film.read(), st, film
