@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -106,6 +108,119 @@ def cpu_baseline(name: str, seconds: float, max_procs: int = 16):
 
 
 # ---------------------------------------------------------------------------------------------
+# Guided versus plain: time to reach a target relative MSE on the guided workload (the icosphere
+# geometry filled with the homogeneous medium). Both estimators sample Lambertian directions in an
+# orthonormal frame, so they converge to the same film; the reference film is a long plain render.
+
+TRIM = 1e-3  # fraction of per-element errors left out of the trimmed relMSE
+
+
+def guided_scene(width: int, height: int):
+    s = scenes.icosphere_box(level=6, width=width, height=height, time_bins=1024, exposure=0.01, sqrt_spp=8,
+                             n_incoming=1, max_bounces=16, seed=0)
+    s.medium = [0.05, 0.05, 0.05, 0.5, 0.5, 0.5]
+    return s
+
+
+def rel_mse(x, r, eps_scale=1e-2, trim=0.0):
+    """mean over (pixel, bin, channel) of (x - r)^2 / (r^2 + eps), eps = eps_scale * mean(r)^2;
+    with trim > 0 the largest `trim` fraction of the per-element errors is left out."""
+    eps = eps_scale * float(np.mean(r)) ** 2
+    e = ((x.astype(np.float64) - r) ** 2 / (r.astype(np.float64) ** 2 + eps)).ravel()
+    if trim > 0:
+        k = int(len(e) * (1.0 - trim))
+        e = np.partition(e, k)[:k]
+    return float(np.mean(e))
+
+
+def guided_comparison(width=128, height=128, ref_spp=65536, pass_sqrt=8, checkpoints=(1, 2, 4, 8, 16, 32),
+                      guide_wave=1 << 18, steps_per_wave=4):
+    from jarabotransientrendering_amd import renderer
+    s = guided_scene(width, height)
+    world, cam, sampler, film, integ = renderer.setup_scene(s)
+    eng = renderer.TransientRenderer()
+    eng.orthonormal_frame = True
+    # reference: plain passes of 32x32 camera samples until ref_spp
+    sampler.sqrt_spp = 32
+    t0 = time.perf_counter()
+    n_ref = max(1, ref_spp // 1024)
+    for k in range(n_ref):
+        eng.render(None, world, integ, cam, film, sampler, seed=100000 + k)
+    ref = film.read().astype(np.float64)
+    ref_sec = time.perf_counter() - t0
+    sampler.sqrt_spp = pass_sqrt
+    pass_paths = s.n_paths() // (s.render.sqrt_spp ** 2) * pass_sqrt ** 2
+
+    def run(guide):
+        film.clean()
+        eng.wave_paths = guide_wave if guide is not None else 0
+        t_render, out, done = 0.0, [], 0
+        for cp in checkpoints:
+            while done < cp:
+                st = eng.render(None, world, integ, cam, film, sampler, seed=done, guide=guide)
+                t_render += st["render_ms"] * 1e-3
+                done += 1
+            img = film.read()
+            out.append({"passes": cp, "spp": cp * pass_sqrt ** 2, "seconds": t_render,
+                        "rel_mse": rel_mse(img, ref), "rel_mse_trim": rel_mse(img, ref, trim=TRIM)})
+        return out
+
+    plain = run(None)
+    guide = renderer.PathGuide(steps_per_wave=steps_per_wave)
+    guided = run(guide)
+    # Next-event estimation at vertices close to the light has unbounded variance (irradiance ~ 1/d^2,
+    # in the reference as here), so the plain relMSE is dominated by single samples; the target and
+    # the times use the relMSE without the largest 0.1 % of per-element errors, and both are reported.
+    key = "rel_mse_trim"
+    target = plain[-1][key]
+
+    def time_to(curve):
+        prev = None
+        for c in curve:
+            if c[key] <= target:
+                if prev is None:
+                    return c["seconds"]
+                # log-log interpolation between checkpoints
+                a = (np.log(target) - np.log(prev[key])) / (np.log(c[key]) - np.log(prev[key]))
+                return float(np.exp(np.log(prev["seconds"]) + a * (np.log(c["seconds"]) - np.log(prev["seconds"]))))
+            prev = c
+        # not reached: extrapolate with relMSE ~ 1/time from the last checkpoint
+        return float(curve[-1]["seconds"] * curve[-1][key] / target)
+
+    t_plain, t_guided = time_to(plain), time_to(guided)
+    return {
+        "workload": (f"{s.n_tris}-triangle displaced icosphere in the open unit box filled with a homogeneous "
+                     f"medium (sigma_a 0.05, sigma_s 0.5), 0.3x0.3 area light; {width}x{height} px x 1024 bins "
+                     f"of 0.01; max 16 events; passes of {pass_sqrt ** 2} spp ({pass_paths} paths)"),
+        "reference": {"spp": n_ref * 1024, "seconds": ref_sec, "estimator": "plain, orthonormal frame"},
+        "rel_mse_eps": "1e-2 * mean(reference)^2", "target_metric": "rel_mse_trim (largest 0.1 % of per-element errors left out)",
+        "target_rel_mse": target,
+        "seconds_to_target": {"plain": t_plain, "guided": t_guided},
+        "speedup": t_plain / t_guided,
+        "plain": plain,
+        "guided": guided,
+        "guide": guide.info(),
+    }
+
+
+def copy_probe(nbytes: int, reps: int = 3) -> None:
+    """Device-to-device copies of a known size (the HIP runtime's copy kernel), the calibration
+    workload for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise SystemExit("copy probe: hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    for _ in range(reps):
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise SystemExit("copy probe: hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -118,6 +233,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--binary-bvh", action="store_true", help="traverse the binary BVH (default: 4-wide)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of BYTES (counter calibration)")
+    ap.add_argument("--guided", choices=["on", "off", "only"], default="off",
+                    help="add the guided-versus-plain time-to-relMSE comparison (only: print just that)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures a single MI355X; --gpus must be 1 (the multi-GPU split is a "
@@ -127,6 +246,15 @@ def main():
         print("bench.py: run as a single process", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    if args.guided == "only":
+        res = guided_comparison()
+        print(json.dumps({"metric": "seconds to target relative MSE, guided vs plain", "value": res["speedup"],
+                          "unit": "x (plain time / guided time)", "n_gpus": 1, "higher_is_better": True,
+                          "dtype": "f32 tracer, bf16 network", "data": "synthetic", "guided": res}))
+        return
     scene, desc = workload(args.workload)
     cpu = None
     if not args.no_cpu_baseline:
@@ -204,6 +332,8 @@ def main():
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
+    if args.guided == "on":
+        out["guided"] = guided_comparison()
     print(json.dumps(out))
 
 
