@@ -5,7 +5,7 @@ A step is one full render pass of the workload (every path of every pixel,
 every bounce, the film accumulation and its compensated merge) with the scene
 and film already resident in HBM. Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload icosphere6|box|media]
+    python bench.py [--steps K] [--warmup W] [--workload icosphere6|box|media|batch64] [--guided on|only]
 
 The CPU baseline (the oracle, oracle/oracle.c, same RNG scheme and workload)
 runs first, on a bounded sample of rows, before anything touches the GPU.
@@ -53,6 +53,10 @@ def workload(name: str):
                              n_incoming=1, max_bounces=16, seed=0)
         desc = ("box filled with a homogeneous isotropic medium (sigma_s 0.5, sigma_a 0.05), point light; "
                 "scanline 256 of 512x512, 1024 bins of 0.01; 4096 spp; max 16 events")
+    elif name.startswith("batch64:"):
+        k = int(name.split(":")[1])
+        s = scenes.batch_scenes(1, k)[0]
+        desc = f"job {k} of the 64-scene batched workload"
     else:
         raise SystemExit(f"unknown workload {name}")
     return s, desc
@@ -203,6 +207,115 @@ def guided_comparison(width=128, height=128, ref_spp=65536, pass_sqrt=8, checkpo
     }
 
 
+# ---------------------------------------------------------------------------------------------
+# Batched workload: 64 uneven scenes (seeds 0..63) rendered together by one launch sequence.
+
+def cpu_baseline_batch(n_jobs: int, seconds: float, max_procs: int = 16):
+    """The oracle on host cores over a bounded sample of the batched workload: process p renders
+    the middle row of job p (of the first min(cores, 16) jobs) at a reduced sample count."""
+    from oracle import pyoracle as po
+    if not os.path.exists(po.LIB_PATH):
+        po.build()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    procs = max(1, min(cores, max_procs, n_jobs))
+    jobs = []
+    for p in range(procs):
+        s, _ = workload(f"batch64:{p}")
+        H, W = s.film.height, s.film.width
+        paths1, sec1 = _cpu_worker((f"batch64:{p}", (H // 2, H // 2 + 1), 1, 99))  # calibration
+        rate = paths1 / max(sec1, 1e-9)
+        spp = max(1, int(round((rate * seconds / W) ** 0.5)))
+        jobs.append((f"batch64:{p}", (H // 2, H // 2 + 1), spp, p))
+    ctx = mp.get_context("fork")  # no GPU state exists yet in this process
+    t0 = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    paths = sum(r[0] for r in res)
+    per_core = paths / sum(r[1] for r in res)
+    sample = (f"{procs} processes; process p renders the middle row of batched job p at a reduced sample count "
+              f"(per-path RNG streams, as on the GPU): {paths} paths in {wall:.1f} s wall; per core "
+              f"{per_core:.4g} paths/s of render loop")
+    return {"value": paths / wall, "unit": "paths/s", "cores": procs, "kind": "port", "sample": sample,
+            "value_per_core": per_core}
+
+
+def bench_batch(args) -> dict:
+    n_jobs = 64
+    cpu = None if args.no_cpu_baseline else cpu_baseline_batch(n_jobs, args.cpu_seconds)
+    from jarabotransientrendering_amd import renderer
+    t0 = time.perf_counter()
+    scs = scenes.batch_scenes(n_jobs)
+    setups = [renderer.setup_scene(s) for s in scs]
+    build_s = time.perf_counter() - t0
+    b = renderer.Batch()
+    for (w, cam, sampler, film, integ), s in zip(setups, scs):
+        b.add(w, integ, cam, film, sampler, seed=s.render.seed)
+    tuned, tune_ms = b.autotune()
+    launch = dict(tuned)
+    for k in range(args.warmup):
+        b.render(launch)
+    for (_, _, _, film, _) in setups:
+        film.clean()
+    steps = []
+    t1 = time.perf_counter()
+    for k in range(args.steps):
+        steps.append(b.render(launch))
+    wall = time.perf_counter() - t1
+    paths = sum(s["paths"] for s in steps)
+    value = paths / wall
+    # one profiled batch pass without graph replay for the per-kernel split
+    prof = b.render({**launch, "flags": launch["flags"] & ~8 | 1})
+    kern_ms, kern_n = prof["kernel_ms"], prof["kernel_launches"]
+    alg = {"extend": BYTES_EXTEND_PER_RAY * prof["extension_rays"],
+           "shadow": BYTES_SHADOW_PER_RAY * prof["shadow_rays"],
+           "shade": BYTES_SHADE_PER_SEGMENT * prof["extension_rays"],
+           "film": BYTES_FILM_PER_RECORD * prof["shadow_rays"] + BYTES_FILM_PER_CONTRIB * prof["contributions"],
+           "camera": 80 * prof["paths"],
+           "film_merge": sum(2 * 4 * f.rows * f.width * f.time_bins * 3 for (_, _, _, f, _) in setups)}
+    dom = max(kern_ms, key=lambda n: kern_ms[n])
+    avg_ms = kern_ms[dom] / max(kern_n[dom], 1)
+    bpl = alg[dom] / max(kern_n[dom], 1)
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    # the same jobs one render call at a time
+    eng = renderer.TransientRenderer()
+    t2 = time.perf_counter()
+    seq_paths = 0
+    for (w, cam, sampler, film, integ), s in zip(setups, scs):
+        seq_paths += eng.render(None, w, integ, cam, film, sampler, seed=s.render.seed)["paths"]
+    seq_wall = time.perf_counter() - t2
+    ext = sum(s["extension_rays"] for s in steps)
+    path_gbs = BYTES_PATH_PER_SEGMENT * ext / wall / 1e9
+    params = [scenes.batch_scene_params(k) for k in range(n_jobs)]
+    return {
+        "metric": "time-resolved paths/s on 1 MI355X",
+        "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": ("batch64: 64 icosphere-in-box scenes (seeds 0..63; triangle target, film size, "
+                                "time bins and spp drawn log-uniform per seed; nearest icosphere level), "
+                                "max 8 bounces, rendered as one batched launch sequence"),
+                   "paths_per_step": paths // args.steps,
+                   "triangles_total": int(sum(s.n_tris for s in scs)),
+                   "film_bins_total": int(sum(p["size"] ** 2 * p["time_bins"] for p in params)),
+                   "parallelism": "single GPU", "launch": launch, "autotune_trial_ms": tune_ms,
+                   "scene_build_s": build_s},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
+                     "launches": kern_n[dom], "algorithmic_bytes_per_launch": bpl},
+        "path_roofline": {"bytes_per_segment": BYTES_PATH_PER_SEGMENT, "segments_per_path": ext / paths,
+                          "achieved": path_gbs, "frac": path_gbs / HBM_PEAK_GBS},
+        "kernels_ms": {k: round(v, 3) for k, v in kern_ms.items()},
+        "sequential": {"paths_per_s": seq_paths / seq_wall, "seconds": seq_wall,
+                       "note": "the same 64 jobs rendered one tr_render_transient call at a time"},
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+    }
+
+
 def copy_probe(nbytes: int, reps: int = 3) -> None:
     """Device-to-device copies of a known size (the HIP runtime's copy kernel), the calibration
     workload for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -254,6 +367,9 @@ def main():
         print(json.dumps({"metric": "seconds to target relative MSE, guided vs plain", "value": res["speedup"],
                           "unit": "x (plain time / guided time)", "n_gpus": 1, "higher_is_better": True,
                           "dtype": "f32 tracer, bf16 network", "data": "synthetic", "guided": res}))
+        return
+    if args.workload == "batch64":
+        print(json.dumps(bench_batch(args)))
         return
     scene, desc = workload(args.workload)
     cpu = None

@@ -135,6 +135,14 @@ int tr_film_read(tr_film film, float* out, float* steady);
 int tr_film_read_sums(tr_film film, double* sums, double* weights);
 /* Device pointers of the compensated-sum buffers (read-only use by the caller). */
 int tr_film_device_buffers(tr_film film, float** d_sum, float** d_comp);
+/* StreakCameraFilm::write (src/Film/StreakCameraFilm.h:233-242, 357-437) and Film::write
+ * (src/Film/Film.h:358-471) with Imaging::save (src/Image/ImageIO.h:147-194): one streak image
+ * per scanline y, "<name>_%04d.<ext>" (T wide, W high; stored transposed when T < 8), ext "hdr"
+ * (Radiance RGBE) or "raw" (FP32 dump), then the time-integrated steady image "<name>.hdr". */
+int tr_film_write(tr_film film, const char* name, const char* ext);
+/* Imaging::save (src/Image/ImageIO.h:147-194) for an RGB float image [height][width][3]: the
+ * extension of `path` selects Radiance RGBE (.hdr) or a raw FP32 dump (.raw). Host only. */
+int tr_image_save(const char* path, const float* rgb, int width, int height);
 
 /* ---- rendering (TransientRenderer::render) ------------------------------------------------ */
 
@@ -162,7 +170,10 @@ enum {
     /* sample Lambertian directions in an orthonormal frame around the normal instead of the
      * reference's frame (Intersection.h:167-178 builds a tangent that is not orthogonal to n
      * when n.z != 0); guided renders always use the orthonormal frame */
-    TR_RENDER_ORTHONORMAL_FRAME = 4u
+    TR_RENDER_ORTHONORMAL_FRAME = 4u,
+    /* capture the kernels of one wavefront (camera + every bounce) in a hipGraph once per call
+     * and replay it for each wavefront; per-kernel timing is not available in this mode */
+    TR_RENDER_GRAPH = 8u
 };
 
 enum {
@@ -187,6 +198,37 @@ typedef struct {
  * per path and per path for light sampling, all seeded from desc->seed. */
 int tr_render_transient(tr_scene scene, const tr_camera* camera, tr_film film,
                         const tr_render_desc* desc, tr_render_stats* stats);
+
+/* ---- batched renders (no counterpart in the reference) -------------------------------------
+ * Many render jobs (scene, camera, film, sampling), typically of uneven size, rendered by ONE
+ * sequence of wavefronts: paths of all jobs share the persistent traversal and shading kernels,
+ * and each path looks up its job, so small jobs do not leave the GPU idle. Scenes and films are
+ * referenced, not copied: keep them alive until tr_batch_render returns. Each job accumulates
+ * into its own film as tr_render_transient would (films must be distinct). */
+typedef struct tr_batch_s* tr_batch;
+
+typedef struct {
+    uint32_t wave_paths;    /* paths per wavefront (0 = library default) */
+    uint32_t flags;         /* TR_RENDER_PROFILE | TR_RENDER_BINARY_BVH | TR_RENDER_ORTHONORMAL_FRAME | TR_RENDER_GRAPH */
+    uint32_t trace_per_cu;  /* persistent traversal blocks per compute unit (0 = default 16) */
+    uint32_t shade_per_cu;  /* shading blocks per compute unit (0 = default 2) */
+} tr_launch_desc;
+
+int tr_batch_create(tr_batch* out);
+int tr_batch_destroy(tr_batch batch);
+/* Appends a job; `job` (optional) receives its index. Jobs render in the order added. */
+int tr_batch_add(tr_batch batch, tr_scene scene, const tr_camera* camera, tr_film film,
+                 const tr_render_desc* desc, uint32_t* job);
+int tr_batch_clear(tr_batch batch);
+/* Renders every job; `launch` may be null (defaults, or the last tr_autotune result).
+ * stats aggregates all jobs. */
+int tr_batch_render(tr_batch batch, const tr_launch_desc* launch, tr_render_stats* stats);
+
+/* Autotuner: renders the batch's jobs at one camera sample per pixel under each candidate launch
+ * shape (wavefront size x traversal blocks per CU x shading blocks per CU, with and without
+ * graph replay), keeps the fastest for later tr_batch_render calls and returns it in `best`
+ * (optional). The films are cleared afterwards. `ms_best` (optional) receives its time. */
+int tr_autotune(tr_batch batch, tr_launch_desc* best, double* ms_best);
 
 /* ---- path guiding (no counterpart in the reference) ---------------------------------------
  * A 48 -> 64 -> 64 -> 64 -> 64 -> 40 ReLU network (BF16 weights, FP32 accumulation on the matrix

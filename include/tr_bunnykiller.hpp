@@ -212,24 +212,14 @@ public:
         check(tr_film_read(handle(), out.data(), nullptr), "StreakCameraFilm::data");
         return out;
     }
-    // StreakCameraFilm::write (StreakCameraFilm.h:233-242, 399-437): one float32 image of
-    // width T and height W per scanline, "<name>_%04d.raw"
+    // StreakCameraFilm::write + Film::write (StreakCameraFilm.h:233-242, 357-437; Film.h:358-471):
+    // "<name>_%04d.<ext>" per scanline (T wide, W high; transposed when T < 8), ext "hdr" (RGBE)
+    // or "raw" (FP32), then the steady image "<name>.hdr"
     void write(const char* name = nullptr)
     {
         std::string base = name ? name : name_;
         if (base.empty()) throw std::runtime_error("StreakCameraFilm::write: no name");
-        std::vector<float> all = data();
-        const size_t slice = size_t(d_.width) * d_.time_bins * 3;
-        for (int r = 0; r < rows(); ++r) {
-            int y = d_.scanline >= 0 ? d_.scanline : r;
-            char fname[4096];
-            std::snprintf(fname, sizeof fname, "%s_%04d.raw", base.c_str(), y);
-            FILE* f = std::fopen(fname, "wb");
-            if (!f) throw std::runtime_error(std::string("Error: failed to open output file: \"") + fname + "\"");
-            size_t wrote = std::fwrite(all.data() + r * slice, sizeof(float), slice, f);
-            std::fclose(f);
-            if (wrote != slice) throw std::runtime_error(std::string("Error: failed to write to output file: ") + fname);
-        }
+        check(tr_film_write(handle(), base.c_str(), ext_.c_str()), "StreakCameraFilm::write");
     }
 
 private:

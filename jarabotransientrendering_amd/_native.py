@@ -21,6 +21,7 @@ TR_ERR_STATE = -4
 TR_RENDER_PROFILE = 1
 TR_RENDER_BINARY_BVH = 2
 TR_RENDER_ORTHONORMAL_FRAME = 4
+TR_RENDER_GRAPH = 8
 K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FILM_MERGE, K_FILM, K_GUIDE = 0, 1, 2, 3, 4, 5, 6
 K_NAMES = {K_CAMERA: "camera", K_EXTEND: "extend", K_SHADE: "shade", K_SHADOW: "shadow",
            K_FILM_MERGE: "film_merge", K_FILM: "film", K_GUIDE: "guide"}
@@ -89,6 +90,14 @@ class RenderStats(ctypes.Structure):
         return d
 
 
+class LaunchDesc(ctypes.Structure):
+    _fields_ = [("wave_paths", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("trace_per_cu", ctypes.c_uint32),
+                ("shade_per_cu", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class GuideDesc(ctypes.Structure):
     _fields_ = [("learning_rate", ctypes.c_float), ("bsdf_fraction", ctypes.c_float),
                 ("train_paths", ctypes.c_uint32), ("batch_size", ctypes.c_uint32),
@@ -122,8 +131,17 @@ SIGNATURES = {
     "tr_film_read_sums": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "tr_film_device_buffers": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                               ctypes.POINTER(ctypes.c_void_p)]),
+    "tr_film_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]),
+    "tr_image_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "tr_render_transient": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CameraDesc), ctypes.c_void_p,
                                            ctypes.POINTER(RenderDesc), ctypes.POINTER(RenderStats)]),
+    "tr_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "tr_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "tr_batch_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CameraDesc), ctypes.c_void_p,
+                                    ctypes.POINTER(RenderDesc), ctypes.POINTER(ctypes.c_uint32)]),
+    "tr_batch_clear": (ctypes.c_int, [ctypes.c_void_p]),
+    "tr_batch_render": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LaunchDesc), ctypes.POINTER(RenderStats)]),
+    "tr_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LaunchDesc), ctypes.POINTER(ctypes.c_double)]),
     "tr_guide_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(GuideDesc)]),
     "tr_guide_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "tr_guide_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

@@ -76,3 +76,11 @@ int fail(int code, const std::string& msg);  // records the message for tr_last_
 int check_device();                          // TR_OK if the current device is a gfx950
 int compute_units();                         // compute units of the current device
 }  // namespace tri
+
+// ---- image output (tr_image_io.cpp) ---------------------------------------------------------
+namespace tri {
+void rgbe_from_float(float r, float g, float b, uint8_t out[4]);
+// RGB float image, row-major [height][width][3], as a Radiance RGBE file (run-length encoded rows)
+int write_rgbe(const char* path, const float* rgb, int width, int height);
+int write_raw(const char* path, const float* data, size_t count);
+}  // namespace tri

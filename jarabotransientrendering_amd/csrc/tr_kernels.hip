@@ -108,24 +108,69 @@ struct WaveDev {
     float4* sh_c;     // contribution rgb, w = film pixel (uint bits)
     uint8_t* sh_vis;  // 1 if the shadow ray is unoccluded
     uint32_t* sh_tag; // guided renders: training record of the contribution (or ~0)
+    uint32_t* job;    // batched renders: job of each path slot
+    uint32_t* sh_job; // batched renders: job of each shadow record
+    struct WaveCtl* ctl;
     uint32_t* counters;
 };
+
+// One render job of a batched launch: its scene, film/camera set-up and first global path index.
+struct JobDev {
+    SceneDev S;
+    RenderDev R;
+    uint64_t path_begin;
+};
+
+// Paths of the current wavefront: [base, base + n) of the launch's global path range. Kept in
+// device memory so that the bounce loop of a wavefront can be replayed from a captured graph.
+struct WaveCtl {
+    uint64_t base;
+    uint32_t n;
+    uint32_t pad;
+};
+
+__global__ void set_wave_kernel(WaveCtl* ctl, uint64_t base, uint32_t n)
+{
+    ctl->base = base;
+    ctl->n = n;
+    ctl->pad = 0u;
+}
 
 __device__ __forceinline__ uint64_t u64(uint32_t lo, uint32_t hi) { return uint64_t(lo) | (uint64_t(hi) << 32); }
 
 // ---------------------------------------------------------------------------------------------
 // camera: one thread per path of the wavefront
-__global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S, RenderDev R, WaveDev Wv, uint32_t wave_base,
-                                                        uint32_t n)
+template <bool BATCH>
+__global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S0, RenderDev R0, const JobDev* __restrict__ jobs,
+                                                        uint32_t n_jobs, WaveDev Wv)
 {
+    const WaveCtl ctl = *Wv.ctl;
+    const uint32_t n = ctl.n;
     uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot == 0) {
         Wv.counters[C_Q0] = n;
         Wv.counters[C_FETCH_EXT] = 0;
     }
     if (slot >= n) return;
-    // a render call holds fewer than 2^32 paths: 32-bit index arithmetic
-    uint32_t pid_local = wave_base + slot;
+    // a job holds fewer than 2^32 paths: 32-bit index arithmetic within the job
+    uint32_t jid = 0, pid_local;
+    if constexpr (BATCH) {
+        // job of this path: the last job whose first path is <= the global path index
+        const uint64_t gidx = ctl.base + slot;
+        uint32_t lo = 0, hi = n_jobs;
+        while (hi - lo > 1u) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (jobs[mid].path_begin <= gidx) lo = mid;
+            else hi = mid;
+        }
+        jid = lo;
+        pid_local = uint32_t(gidx - jobs[lo].path_begin);
+        Wv.job[slot] = jid;
+    } else {
+        pid_local = uint32_t(ctl.base) + slot;
+    }
+    const SceneDev& S = BATCH ? jobs[jid].S : S0;
+    const RenderDev& R = BATCH ? jobs[jid].R : R0;
     uint32_t cs_local = pid_local / uint32_t(R.n_inc);
     uint32_t inc_i = pid_local - cs_local * uint32_t(R.n_inc);
     uint32_t pix_local = cs_local / uint32_t(R.spp2);   // film pixel (row relative)
@@ -169,8 +214,9 @@ __global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S, RenderDev R,
 
 // ---------------------------------------------------------------------------------------------
 // extend: closest hit for each queued ray
-template <bool WIDE, int STACK>
-__global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S, WaveDev Wv, int cur)
+template <bool WIDE, int STACK, bool BATCH>
+__global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv,
+                                                             int cur)
 {
     __shared__ int32_t stack[STACK * kTraceBlock];
     const uint32_t count = Wv.counters[cur];
@@ -194,6 +240,7 @@ __global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S, WaveDev
         const uint32_t i = base + j + lane;
         if (i < count) {
             uint32_t slot = Wv.q[cur][i];
+            const SceneDev& S = BATCH ? jobs[Wv.job[slot]].S : S0;
             float4 o = Wv.ray_o[slot], d = Wv.ray_d[slot];
             RayPre r = ray_pre(mk(o), mk(d));
             float tfar = __int_as_float(0x7f800000);
@@ -296,10 +343,11 @@ constexpr int kShadeBlockGuided = 256;  // 4 wavefronts x 10 KiB of LDS for the 
 
 __device__ __forceinline__ float mean3(f3 v) { return (v.x + v.y + v.z) * (1.0f / 3.0f); }
 
-template <int MODE>
+template <int MODE, bool BATCH>
 __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kShadeBlock)
-    shade_kernel(SceneDev S, RenderDev R, WaveDev Wv, int cur, trg::GuideView G)
+    shade_kernel(SceneDev S0, RenderDev R0, const JobDev* __restrict__ jobs, WaveDev Wv, int cur, trg::GuideView G)
 {
+    static_assert(!(BATCH && MODE == kShadeGuided), "guided renders take one job");
     constexpr bool GUIDED = MODE == kShadeGuided;
     __shared__ uint32_t scan[kShadeBlock / 64 + 1];
     __shared__ float glds[GUIDED ? (kShadeBlockGuided / 64) * 64 * trg::kRow : 1];
@@ -318,6 +366,9 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
         const uint32_t i = base + threadIdx.x;
         bool live = i < count;
         uint32_t slot = live ? Wv.q[cur][i] : 0u;
+        const uint32_t jid = (BATCH && live) ? Wv.job[slot] : 0u;
+        const SceneDev& S = BATCH ? jobs[jid].S : S0;
+        const RenderDev& R = BATCH ? jobs[jid].R : R0;
         bool emit_shadow = false, cont = false, want = false;
         float4 sh_o = {}, sh_d = {}, sh_c = {};
         float4 n_o = {}, n_d = {}, n_thr = {};
@@ -562,6 +613,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
             Wv.sh_d[sh_pos] = sh_d;
             Wv.sh_c[sh_pos] = sh_c;
             if constexpr (GUIDED) Wv.sh_tag[sh_pos] = sh_tag;
+            if constexpr (BATCH) Wv.sh_job[sh_pos] = jid;
         }
         uint32_t q_pos = block_append(Wv.counters + (cur ^ 1), cont ? 1u : 0u, scan);
         if (cont) {
@@ -577,8 +629,8 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
 
 // ---------------------------------------------------------------------------------------------
 // shadow: occlusion test light -> vertex in (eps, dist - eps]; visible contributions go to the film
-template <bool WIDE, int STACK>
-__global__ void __launch_bounds__(kTraceBlock) shadow_kernel(SceneDev S, WaveDev Wv)
+template <bool WIDE, int STACK, bool BATCH>
+__global__ void __launch_bounds__(kTraceBlock) shadow_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv)
 {
     __shared__ int32_t stack[STACK * kTraceBlock];
     const uint32_t count = Wv.counters[C_SHADOW];
@@ -596,6 +648,7 @@ __global__ void __launch_bounds__(kTraceBlock) shadow_kernel(SceneDev S, WaveDev
         for (uint32_t j = 0; j < chunk; j += 64) {
             const uint32_t i = base + j + lane;
             if (i < count) {
+                const SceneDev& S = BATCH ? jobs[Wv.sh_job[i]].S : S0;
                 float4 o = Wv.sh_o[i], d = Wv.sh_d[i];
                 RayPre r = ray_pre(mk(o), mk(d));
                 float tfar = o.w - kSigmaVisibility;
@@ -677,6 +730,23 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
     }
 }
 
+// batched renders: visible contributions go straight to their job's pass film with float atomics
+__global__ void __launch_bounds__(kBlock) film_batch_kernel(const JobDev* __restrict__ jobs, WaveDev Wv)
+{
+    const uint32_t count = Wv.counters[C_SHADOW];
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count; r += gridDim.x * blockDim.x) {
+        if (!Wv.sh_vis[r]) continue;
+        const RenderDev& R = jobs[Wv.sh_job[r]].R;
+        float4 c = Wv.sh_c[r];
+        uint32_t pix = __float_as_uint(c.w);
+        uint32_t bin = __float_as_uint(Wv.sh_d[r].w) - pix * uint32_t(R.T);
+        float* dst = R.film + (size_t(pix) * uint32_t(R.T) + bin) * 3;
+        unsafeAtomicAdd(dst + 0, c.x);
+        unsafeAtomicAdd(dst + 1, c.y);
+        unsafeAtomicAdd(dst + 2, c.z);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // film merge: Neumaier-compensated accumulation of the pass buffer into (sum, comp); clears the pass
 __global__ void __launch_bounds__(kBlock) film_merge_kernel(float4* __restrict__ pass, float4* __restrict__ sum,
@@ -742,6 +812,20 @@ __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, cons
         else if ((bound) < 32) KERNEL<W, 32><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);     \
         else KERNEL<W, kBvhStack><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);                \
     } while (0)
+#define TR_STACK_DISPATCH3(W, bound, B, KERNEL, GRID, BLOCK, STREAM, ...)                \
+    do {                                                                                   \
+        if ((bound) < 16) KERNEL<W, 16, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);       \
+        else if ((bound) < 24) KERNEL<W, 24, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);  \
+        else if ((bound) < 32) KERNEL<W, 32, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);  \
+        else KERNEL<W, kBvhStack, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);             \
+    } while (0)
+#define TR_TRACE_DISPATCH(wide, bound, batch, KERNEL, GRID, BLOCK, STREAM, ...)                    \
+    do {                                                                                           \
+        if (wide && batch) TR_STACK_DISPATCH3(true, bound, true, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);    \
+        else if (wide) TR_STACK_DISPATCH3(true, bound, false, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);       \
+        else if (batch) TR_STACK_DISPATCH3(false, bound, true, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);      \
+        else TR_STACK_DISPATCH3(false, bound, false, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);                \
+    } while (0)
 #define TR_STACK_DISPATCH(wide, s, KERNEL, GRID, BLOCK, STREAM, ...)                       \
     do {                                                                                   \
         if (wide) TR_STACK_DISPATCH_W(true, (s)->stack4, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);  \
@@ -801,6 +885,17 @@ void dev_free(T*& p)
     p = nullptr;
 }
 
+// Launch shape and scheduling choices of a render (the autotuner searches over these).
+struct LaunchCfg {
+    uint32_t wave = kDefaultWave;   // paths per wavefront
+    uint32_t trace_per_cu = 16;     // persistent traversal blocks per compute unit
+    uint32_t shade_per_cu = 2;      // shade blocks (1024 threads) per compute unit
+    bool wide = true;               // 4-wide BVH
+    bool graph = false;             // replay each wavefront's bounce loop from a captured hipGraph
+    bool prof = false;              // per-kernel HIP events
+    int mode = kShadeRef;
+};
+
 struct WaveBuffers {
     uint32_t cap = 0;
     float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr;
@@ -810,14 +905,30 @@ struct WaveBuffers {
     float4 *sh_o = nullptr, *sh_d = nullptr, *sh_c = nullptr;
     uint8_t* sh_vis = nullptr;
     uint32_t* sh_tag = nullptr;  // allocated by the first guided render
+    uint32_t *job = nullptr, *sh_job = nullptr;  // allocated by the first batched render
+    WaveCtl* ctl = nullptr;
+    JobDev* d_jobs = nullptr;
+    uint32_t jobs_cap = 0;
     uint32_t* counters = nullptr;
 
     void release()
     {
         dev_free(ray_o); dev_free(ray_d); dev_free(thr); dev_free(misc); dev_free(extra); dev_free(hit);
         dev_free(q0); dev_free(q1); dev_free(sh_o); dev_free(sh_d); dev_free(sh_c); dev_free(sh_vis); dev_free(counters);
-        dev_free(sh_tag);
+        dev_free(sh_tag); dev_free(job); dev_free(sh_job); dev_free(ctl); dev_free(d_jobs);
         cap = 0;
+        jobs_cap = 0;
+    }
+    int ensure_batch(uint32_t n_jobs)
+    {
+        int rc;
+        if (!job && cap && ((rc = dev_alloc(&job, cap)) || (rc = dev_alloc(&sh_job, cap)))) return rc;
+        if (n_jobs + 1 > jobs_cap) {
+            dev_free(d_jobs);
+            if ((rc = dev_alloc(&d_jobs, n_jobs + 1))) return rc;
+            jobs_cap = n_jobs + 1;
+        }
+        return TR_OK;
     }
     int ensure_tags()
     {
@@ -833,7 +944,7 @@ struct WaveBuffers {
             (rc = dev_alloc(&misc, n)) || (rc = dev_alloc(&extra, n)) || (rc = dev_alloc(&hit, n)) ||
             (rc = dev_alloc(&q0, n)) || (rc = dev_alloc(&q1, n)) || (rc = dev_alloc(&sh_o, n)) ||
             (rc = dev_alloc(&sh_d, n)) || (rc = dev_alloc(&sh_c, n)) || (rc = dev_alloc(&sh_vis, n)) ||
-            (rc = dev_alloc(&counters, C_WORDS))) {
+            (rc = dev_alloc(&counters, C_WORDS)) || (rc = dev_alloc(&ctl, 1))) {
             release();
             return rc;
         }
@@ -845,6 +956,7 @@ struct WaveBuffers {
         WaveDev w;
         w.ray_o = ray_o; w.ray_d = ray_d; w.thr = thr; w.misc = misc; w.extra = extra; w.hit = hit;
         w.q[0] = q0; w.q[1] = q1; w.sh_o = sh_o; w.sh_d = sh_d; w.sh_c = sh_c; w.sh_vis = sh_vis; w.sh_tag = sh_tag;
+        w.job = job; w.sh_job = sh_job; w.ctl = ctl;
         w.counters = counters;
         return w;
     }
@@ -882,6 +994,7 @@ struct tr_scene_s {
     int depth = 0, depth4 = 0, stack4 = 0;
     bool wide = true;   // traverse the 4-wide BVH (tr_render_desc flags can select the binary one)
     float bb_lo[3] = {0, 0, 0}, bb_hi[3] = {1, 1, 1};  // bounds of the triangles (set at commit)
+    LaunchCfg tune;     // launch shape used by renders of this scene (tr_autotune sets it)
     double build_ms = 0.0;
     uint64_t dev_bytes = 0;
     WaveBuffers wave;
@@ -1301,6 +1414,47 @@ int tr_film_read(tr_film f, float* out, float* steady)
     return TR_OK;
 }
 
+int tr_film_write(tr_film f, const char* name, const char* ext)
+{
+    if (!f || !name || !*name) return fail(TR_ERR_INVALID, "tr_film_write: null film or name");
+    const std::string e = ext ? ext : "hdr";
+    if (e != "hdr" && e != "raw")
+        return fail(TR_ERR_INVALID, "Error: unsupported output file format: \"" + e + "\"");
+    const int W = f->desc.width, T = f->desc.time_bins, H = f->desc.height;
+    std::vector<float> img(f->n_elems), steady(size_t(f->rows) * size_t(W) * 3);
+    int rc = tr_film_read(f, img.data(), steady.data());
+    if (rc) return rc;
+    const size_t slice = size_t(W) * size_t(T) * 3;
+    std::vector<float> tr;
+    char path[4096];
+    for (int r = 0; r < f->rows; ++r) {
+        const int y = f->desc.scanline >= 0 ? f->desc.scanline : r;
+        // a slice is an image T wide and W high; below 8 columns it is stored transposed
+        const float* data = img.data() + size_t(r) * slice;
+        int width = T, height = W;
+        if (T < 8) {
+            tr.resize(slice);
+            for (int x = 0; x < W; ++x)
+                for (int t = 0; t < T; ++t)
+                    for (int c = 0; c < 3; ++c) tr[(size_t(t) * W + x) * 3 + c] = data[(size_t(x) * T + t) * 3 + c];
+            data = tr.data();
+            width = W, height = T;
+        }
+        std::snprintf(path, sizeof path, "%s_%04d.%s", name, y, e.c_str());
+        rc = e == "hdr" ? tri::write_rgbe(path, data, width, height) : tri::write_raw(path, data, slice);
+        if (rc) return rc;
+    }
+    // steady image: W x H, time-integrated radiance (only the streak row in scanline mode)
+    std::vector<float> full(size_t(W) * size_t(H) * 3, 0.0f);
+    for (int r = 0; r < f->rows; ++r) {
+        const int y = f->desc.scanline >= 0 ? f->desc.scanline : r;
+        std::copy(steady.begin() + size_t(r) * W * 3, steady.begin() + size_t(r + 1) * W * 3,
+                  full.begin() + size_t(y) * W * 3);
+    }
+    std::snprintf(path, sizeof path, "%s.hdr", name);
+    return tri::write_rgbe(path, full.data(), W, H);
+}
+
 int tr_film_device_buffers(tr_film f, float** d_sum, float** d_comp)
 {
     if (!f) return fail(TR_ERR_INVALID, "tr_film_device_buffers: null film");
@@ -1313,39 +1467,34 @@ int tr_film_device_buffers(tr_film f, float** d_sum, float** d_comp)
 
 namespace {
 
-int render_impl(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* d, tr_guide guide,
-                tr_render_stats* stats)
+// A render job as the host prepares it.
+struct HostJob {
+    tr_scene scene;
+    tr_film film;
+    JobDev dev;          // scene view, film/camera set-up, first global path index
+    uint64_t paths;
+    uint32_t spp2;
+    int max_bounces;
+};
+
+// Camera and film set-up of one job (PinholeCamera.h:56-80; bunnykiller.cpp:1156-1160).
+int make_job(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* d, HostJob* out, const char* who)
 {
-    auto t_start = std::chrono::steady_clock::now();
-    if (!s || !cam || !f || !d) return fail(TR_ERR_INVALID, "tr_render_transient: null argument");
-    if (!s->committed) return fail(TR_ERR_STATE, "tr_render_transient: scene not committed");
-    if (s->lights.empty()) return fail(TR_ERR_INVALID, "tr_render_transient: scene has no light");
+    if (!s || !cam || !f || !d) return fail(TR_ERR_INVALID, std::string(who) + ": null argument");
+    if (!s->committed) return fail(TR_ERR_STATE, std::string(who) + ": scene not committed");
+    if (s->lights.empty()) return fail(TR_ERR_INVALID, std::string(who) + ": scene has no light");
     if (d->sqrt_spp <= 0 || d->n_incoming <= 0 || d->max_bounces < 0)
-        return fail(TR_ERR_INVALID, "tr_render_transient: bad sampling parameters");
+        return fail(TR_ERR_INVALID, std::string(who) + ": bad sampling parameters");
     const tr_film_desc& fd = f->desc;
     const uint64_t spp2 = uint64_t(d->sqrt_spp) * uint64_t(d->sqrt_spp);
     const uint64_t total = uint64_t(f->rows) * uint64_t(fd.width) * spp2 * uint64_t(d->n_incoming);
     if ((uint64_t(fd.height) * fd.width * spp2 * uint64_t(d->n_incoming)) >= (1ull << 61))
-        return fail(TR_ERR_INVALID, "tr_render_transient: too many paths for the RNG stream ids");
+        return fail(TR_ERR_INVALID, std::string(who) + ": too many paths for the RNG stream ids");
     if (total >= (1ull << 32))
-        return fail(TR_ERR_INVALID, "tr_render_transient: more than 2^32 - 1 paths in one call; split into passes");
+        return fail(TR_ERR_INVALID, std::string(who) + ": more than 2^32 - 1 paths in one job; split into passes");
     if (uint64_t(f->rows) * uint64_t(fd.width) >= (1ull << 24))
-        return fail(TR_ERR_INVALID, "tr_render_transient: film has more than 2^24 pixels");
-    if (d->max_bounces > 255) return fail(TR_ERR_INVALID, "tr_render_transient: max_bounces above 255");
-    uint32_t wave = d->wave_paths ? d->wave_paths : kDefaultWave;
-    wave = uint32_t(std::min<uint64_t>(wave, std::max<uint64_t>(total, 1)));
-    int rc = s->wave.ensure(wave);
-    if (rc) return rc;
-    trg::GuideView GV{};
-    if (guide) {
-        if (d->max_bounces < 1) return fail(TR_ERR_INVALID, "tr_render_transient_guided: max_bounces must be >= 1");
-        if ((rc = s->wave.ensure_tags())) return rc;
-        if ((rc = trg::guide_begin_render(guide, wave, uint32_t(d->max_bounces), s->bb_lo, s->bb_hi, s->stream, &GV)))
-            return rc;
-    }
-    const int mode = guide ? kShadeGuided : ((d->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef);
-
-    // camera set-up (PinholeCamera.h:56-80; bunnykiller.cpp:1156-1160)
+        return fail(TR_ERR_INVALID, std::string(who) + ": film has more than 2^24 pixels");
+    if (d->max_bounces > 255) return fail(TR_ERR_INVALID, std::string(who) + ": max_bounces above 255");
     auto nrm = [](float* v) {
         float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         float inv = float(1. / double(l));
@@ -1391,115 +1540,271 @@ int render_impl(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_des
     R.ratio = float(fd.width) / float(fd.height);
     R.inv_spp = float(1. / double(float(d->sqrt_spp)));
     R.seed = d->seed;
+    out->scene = s;
+    out->film = f;
+    out->dev.S = s->view();
+    out->dev.R = R;
+    out->dev.path_begin = 0;
+    out->paths = total;
+    out->spp2 = uint32_t(spp2);
+    out->max_bounces = d->max_bounces;
+    return TR_OK;
+}
 
-    const bool prof = (d->flags & TR_RENDER_PROFILE) != 0;
-    const bool wide = (d->flags & TR_RENDER_BINARY_BVH) == 0;
-    if (prof && !s->ev_made) {
-        for (auto& e : s->ev) HIP_TRY(hipEventCreate(&e));
-        s->ev_made = true;
-    }
-    std::vector<std::pair<int, int>> ev_kind;  // (kind, event pair index)
+// Everything the kernels of one launch need, shared by the per-wave enqueue.
+struct Launch {
+    hipStream_t stream;
+    WaveBuffers* wb;
+    std::vector<HostJob>* jobs;
+    const LaunchCfg* cfg;
+    tr_guide guide;
+    trg::GuideView GV;
+    int compute_units;
+    hipEvent_t* ev;
+    bool batch;
+    int bounces;
+    int stack_bound;
+    uint32_t pix_slots;
+    size_t film_lds;
+    // profiling
+    std::vector<std::pair<int, int>> ev_kind;
     double kernel_ms[TR_K_COUNT] = {};
     uint32_t launches[TR_K_COUNT] = {};
-    auto flush_events = [&]() -> int {
+    bool events_on = false;
+
+    int flush_events()
+    {
         if (ev_kind.empty()) return TR_OK;
-        HIP_TRY(hipEventSynchronize(s->ev[2 * ev_kind.back().second + 1]));
+        HIP_TRY(hipEventSynchronize(ev[2 * ev_kind.back().second + 1]));
         for (auto& kv : ev_kind) {
             float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, s->ev[2 * kv.second], s->ev[2 * kv.second + 1]));
+            HIP_TRY(hipEventElapsedTime(&ms, ev[2 * kv.second], ev[2 * kv.second + 1]));
             kernel_ms[kv.first] += ms;
         }
         ev_kind.clear();
         return TR_OK;
-    };
-    auto begin = [&](int kind) -> int {
+    }
+    int begin(int kind)
+    {
         launches[kind]++;
-        if (!prof) return TR_OK;
+        if (!events_on) return TR_OK;
         if (ev_kind.size() == 256) {
             int r2 = flush_events();
             if (r2) return r2;
         }
         int slot = int(ev_kind.size());
         ev_kind.push_back({kind, slot});
-        HIP_TRY(hipEventRecord(s->ev[2 * slot], s->stream));
+        HIP_TRY(hipEventRecord(ev[2 * slot], stream));
         return TR_OK;
-    };
-    auto end = [&]() -> int {
+    }
+    int end()
+    {
         HIP_TRY(hipGetLastError());
-        if (!prof) return TR_OK;
-        HIP_TRY(hipEventRecord(s->ev[2 * ev_kind.back().second + 1], s->stream));
+        if (!events_on) return TR_OK;
+        HIP_TRY(hipEventRecord(ev[2 * ev_kind.back().second + 1], stream));
         return TR_OK;
-    };
+    }
 
-    SceneDev S = s->view();
-    WaveDev Wv = s->wave.view();
-    HIP_TRY(hipMemsetAsync(s->wave.counters, 0, C_WORDS * sizeof(uint32_t), s->stream));
-    // the pass buffer is written on the scene stream; order it after any pending film work
-    HIP_TRY(hipStreamSynchronize(f->stream));
-
-    // LDS histogram slots of the film kernel: as many pixels x T bins x RGB as fit in kFilmLdsBytes
-    const uint32_t pix_slots = std::min<uint32_t>(kFilmMaxSlots, kFilmLdsBytes / (uint32_t(fd.time_bins) * 12u));
-    const size_t film_lds = size_t(pix_slots) * size_t(fd.time_bins) * 12u;
-    uint32_t n_waves = 0;
-    int bounces = d->max_bounces;
-    for (uint64_t base = 0; base < total; base += wave) {
-        uint32_t n = uint32_t(std::min<uint64_t>(wave, total - base));
-        ++n_waves;
-        if (guide) {
-            if ((rc = begin(TR_K_GUIDE))) return rc;
-            if ((rc = trg::guide_wave_begin(guide, s->stream))) return rc;
-            if ((rc = end())) return rc;
-        }
+    // the kernels of one wavefront: camera, then per bounce extend / shade / shadow / film
+    int enqueue_wave()
+    {
+        int rc;
+        const LaunchCfg& c = *cfg;
+        const HostJob& j0 = (*jobs)[0];
+        const SceneDev S0 = j0.dev.S;
+        const RenderDev R0 = j0.dev.R;
+        const JobDev* dj = batch ? wb->d_jobs : nullptr;
+        const uint32_t nj = uint32_t(jobs->size());
+        const WaveDev Wv = wb->view();
+        const uint32_t wave = c.wave, cu = uint32_t(std::max(compute_units, 1));
+        const uint32_t tg = std::max(1u, std::min((wave + kTraceBlock - 1) / kTraceBlock, cu * c.trace_per_cu));
+        const uint32_t fg = std::max(1u, std::min((wave + kFilmRange - 1) / kFilmRange, cu * 4u));
+        const uint32_t sblock = c.mode == kShadeGuided ? kShadeBlockGuided : kShadeBlock;
+        const uint32_t scap = cu * (c.mode == kShadeGuided ? 8u : c.shade_per_cu);
+        const uint32_t sg = std::max(1u, std::min((wave + sblock - 1) / sblock, scap));
+        const uint32_t cg = (wave + kBlock - 1) / kBlock;
         if ((rc = begin(TR_K_CAMERA))) return rc;
-        camera_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(S, R, Wv, uint32_t(base), n);
+        if (batch) camera_kernel<true><<<cg, kBlock, 0, stream>>>(S0, R0, dj, nj, Wv);
+        else camera_kernel<false><<<cg, kBlock, 0, stream>>>(S0, R0, dj, nj, Wv);
         if ((rc = end())) return rc;
-        uint32_t tg = persistent_grid(n, s->compute_units);
-        uint32_t fg = std::max(1u, std::min((n + kFilmRange - 1) / kFilmRange, uint32_t(std::max(s->compute_units, 1)) * 4u));
-        const uint32_t sblock = mode == kShadeGuided ? kShadeBlockGuided : kShadeBlock;
-        const uint32_t sgrid_cap = uint32_t(std::max(s->compute_units, 1)) * (mode == kShadeGuided ? 8u : 2u);
-        uint32_t sg = std::max(1u, std::min((n + sblock - 1) / sblock, sgrid_cap));
         int cur = 0;
         for (int b = 0; b < bounces; ++b) {
             if ((rc = begin(TR_K_EXTEND))) return rc;
-            TR_STACK_DISPATCH(wide, s, extend_kernel, tg, kTraceBlock, s->stream, S, Wv, cur);
+            TR_TRACE_DISPATCH(c.wide, stack_bound, batch, extend_kernel, tg, kTraceBlock, stream, S0, dj, Wv, cur);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADE))) return rc;
-            if (mode == kShadeGuided)
-                shade_kernel<kShadeGuided><<<sg, kShadeBlockGuided, 0, s->stream>>>(S, R, Wv, cur, GV);
-            else if (mode == kShadeOnb)
-                shade_kernel<kShadeOnb><<<sg, kShadeBlock, 0, s->stream>>>(S, R, Wv, cur, GV);
-            else
-                shade_kernel<kShadeRef><<<sg, kShadeBlock, 0, s->stream>>>(S, R, Wv, cur, GV);
+            if (batch) {
+                if (c.mode == kShadeOnb)
+                    shade_kernel<kShadeOnb, true><<<sg, kShadeBlock, 0, stream>>>(S0, R0, dj, Wv, cur, GV);
+                else
+                    shade_kernel<kShadeRef, true><<<sg, kShadeBlock, 0, stream>>>(S0, R0, dj, Wv, cur, GV);
+            } else if (c.mode == kShadeGuided) {
+                shade_kernel<kShadeGuided, false><<<sg, kShadeBlockGuided, 0, stream>>>(S0, R0, dj, Wv, cur, GV);
+            } else if (c.mode == kShadeOnb) {
+                shade_kernel<kShadeOnb, false><<<sg, kShadeBlock, 0, stream>>>(S0, R0, dj, Wv, cur, GV);
+            } else {
+                shade_kernel<kShadeRef, false><<<sg, kShadeBlock, 0, stream>>>(S0, R0, dj, Wv, cur, GV);
+            }
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADOW))) return rc;
-            TR_STACK_DISPATCH(wide, s, shadow_kernel, tg, kTraceBlock, s->stream, S, Wv);
+            TR_TRACE_DISPATCH(c.wide, stack_bound, batch, shadow_kernel, tg, kTraceBlock, stream, S0, dj, Wv);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_FILM))) return rc;
-            if (guide) film_kernel<true><<<fg, kFilmBlock, film_lds, s->stream>>>(R, Wv, pix_slots, GV.contrib);
-            else film_kernel<false><<<fg, kFilmBlock, film_lds, s->stream>>>(R, Wv, pix_slots, nullptr);
+            if (batch)
+                film_batch_kernel<<<std::max(1u, std::min((wave + kBlock - 1) / kBlock, cu * 16u)), kBlock, 0, stream>>>(dj, Wv);
+            else if (guide)
+                film_kernel<true><<<fg, kFilmBlock, film_lds, stream>>>(R0, Wv, pix_slots, GV.contrib);
+            else
+                film_kernel<false><<<fg, kFilmBlock, film_lds, stream>>>(R0, Wv, pix_slots, nullptr);
             if ((rc = end())) return rc;
             cur ^= 1;
         }
+        return TR_OK;
+    }
+};
+
+// Renders every job of `jobs` in one sequence of wavefronts on `stream`; jobs share wavefronts
+// when there are several (batched kernels look up each path's job).
+int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, const LaunchCfg& cfg, tr_guide guide,
+             int compute_units, hipEvent_t* ev, bool* ev_made, tr_render_stats* stats,
+             std::chrono::steady_clock::time_point t_start)
+{
+    if (jobs.empty()) return fail(TR_ERR_INVALID, "render: no job");
+    const bool batch = jobs.size() > 1;
+    if (batch && guide) return fail(TR_ERR_INVALID, "render: guided renders take one job");
+    uint64_t total = 0;
+    int bounces = 0, stack_bound = 0;
+    for (auto& j : jobs) {
+        j.dev.path_begin = total;
+        total += j.paths;
+        bounces = std::max(bounces, j.max_bounces);
+        stack_bound = std::max(stack_bound, cfg.wide ? j.scene->stack4 : j.scene->depth);
+    }
+    LaunchCfg c = cfg;
+    c.wave = uint32_t(std::min<uint64_t>(c.wave ? c.wave : kDefaultWave, std::max<uint64_t>(total, 1)));
+    int rc = wb.ensure(c.wave);
+    if (rc) return rc;
+    Launch L{};
+    L.stream = stream;
+    L.wb = &wb;
+    L.jobs = &jobs;
+    L.cfg = &c;
+    L.guide = guide;
+    L.compute_units = compute_units;
+    L.ev = ev;
+    L.batch = batch;
+    L.bounces = bounces;
+    L.stack_bound = stack_bound;
+    if (guide) {
+        const HostJob& j = jobs[0];
+        if (j.max_bounces < 1) return fail(TR_ERR_INVALID, "tr_render_transient_guided: max_bounces must be >= 1");
+        if ((rc = wb.ensure_tags())) return rc;
+        if ((rc = trg::guide_begin_render(guide, c.wave, uint32_t(j.max_bounces), j.scene->bb_lo, j.scene->bb_hi,
+                                          stream, &L.GV)))
+            return rc;
+    }
+    if (batch) {
+        if ((rc = wb.ensure_batch(uint32_t(jobs.size())))) return rc;
+        std::vector<JobDev> dj(jobs.size() + 1);
+        for (size_t k = 0; k < jobs.size(); ++k) dj[k] = jobs[k].dev;
+        dj[jobs.size()] = jobs.back().dev;
+        dj[jobs.size()].path_begin = total;
+        HIP_TRY(hipMemcpyAsync(wb.d_jobs, dj.data(), dj.size() * sizeof(JobDev), hipMemcpyHostToDevice, stream));
+    }
+    if (c.prof && !c.graph) {
+        if (!*ev_made) {
+            for (int k = 0; k < 2 * 256; ++k) HIP_TRY(hipEventCreate(&ev[k]));
+            *ev_made = true;
+        }
+        L.events_on = true;
+    }
+    // LDS histogram slots of the film kernel: as many pixels x T bins x RGB as fit in kFilmLdsBytes
+    {
+        const uint32_t T = uint32_t(jobs[0].dev.R.T);
+        L.pix_slots = std::min<uint32_t>(kFilmMaxSlots, kFilmLdsBytes / (T * 12u));
+        L.film_lds = size_t(L.pix_slots) * size_t(T) * 12u;
+    }
+    HIP_TRY(hipMemsetAsync(wb.counters, 0, C_WORDS * sizeof(uint32_t), stream));
+    // pass buffers are written on this stream; order it after any pending film work
+    for (auto& j : jobs) HIP_TRY(hipStreamSynchronize(j.film->stream));
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto drop_graph = [&]() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    };
+    uint32_t n_waves = 0;
+    for (uint64_t base = 0; base < total; base += c.wave) {
+        uint32_t n = uint32_t(std::min<uint64_t>(c.wave, total - base));
+        ++n_waves;
         if (guide) {
-            if ((rc = begin(TR_K_GUIDE))) return rc;
-            if ((rc = trg::guide_wave_end(guide, s->stream))) return rc;
-            if ((rc = end())) return rc;
+            if ((rc = L.begin(TR_K_GUIDE)) || (rc = trg::guide_wave_begin(guide, stream)) || (rc = L.end())) {
+                drop_graph();
+                return rc;
+            }
+        }
+        set_wave_kernel<<<1, 1, 0, stream>>>(wb.ctl, base, n);
+        if (c.graph) {
+            if (!exec) {
+                uint32_t launches_before[TR_K_COUNT];
+                std::memcpy(launches_before, L.launches, sizeof launches_before);
+                HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                rc = L.enqueue_wave();
+                hipError_t ce = hipStreamEndCapture(stream, &graph);
+                if (rc || ce != hipSuccess) {
+                    drop_graph();
+                    return rc ? rc : fail(TR_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+                }
+                hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (ie != hipSuccess) {
+                    drop_graph();
+                    return fail(TR_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+                }
+                std::memcpy(L.launches, launches_before, sizeof launches_before);
+            }
+            // count the kernels each replay runs
+            L.launches[TR_K_CAMERA] += 1;
+            for (int k : {TR_K_EXTEND, TR_K_SHADE, TR_K_SHADOW, TR_K_FILM}) L.launches[k] += uint32_t(bounces);
+            hipError_t le = hipGraphLaunch(exec, stream);
+            if (le != hipSuccess) {
+                drop_graph();
+                return fail(TR_ERR_DEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(le));
+            }
+        } else if ((rc = L.enqueue_wave())) {
+            return rc;
+        }
+        if (guide) {
+            if ((rc = L.begin(TR_K_GUIDE)) || (rc = trg::guide_wave_end(guide, stream)) || (rc = L.end())) {
+                drop_graph();
+                return rc;
+            }
         }
     }
-    {
+    // each distinct film: pass buffer -> compensated sum
+    std::vector<tr_film> merged;
+    for (auto& j : jobs) {
+        if (std::find(merged.begin(), merged.end(), j.film) != merged.end()) continue;
+        merged.push_back(j.film);
+        tr_film f = j.film;
         size_t n4 = (f->n_elems + 3) / 4;
         uint32_t mg = uint32_t(std::max<size_t>(1, std::min<size_t>((n4 + kBlock - 1) / kBlock, 256u * 64u)));
-        if ((rc = begin(TR_K_FILM_MERGE))) return rc;
-        film_merge_kernel<<<mg, kBlock, 0, s->stream>>>(reinterpret_cast<float4*>(f->d_pass),
-                                                         reinterpret_cast<float4*>(f->d_sum),
-                                                         reinterpret_cast<float4*>(f->d_comp), n4);
-        if ((rc = end())) return rc;
+        if ((rc = L.begin(TR_K_FILM_MERGE))) break;
+        film_merge_kernel<<<mg, kBlock, 0, stream>>>(reinterpret_cast<float4*>(f->d_pass), reinterpret_cast<float4*>(f->d_sum),
+                                                       reinterpret_cast<float4*>(f->d_comp), n4);
+        if ((rc = L.end())) break;
     }
     uint32_t counters[C_WORDS];
-    HIP_TRY(hipMemcpyAsync(counters, s->wave.counters, sizeof counters, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    if ((rc = flush_events())) return rc;
-    for (double& w : f->weight) w += double(spp2);
+    hipError_t e = hipMemcpyAsync(counters, wb.counters, sizeof counters, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    drop_graph();
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TR_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
+    if ((rc = L.flush_events())) return rc;
+    for (auto& j : jobs)
+        for (double& w : j.film->weight) w += double(j.spp2);
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         stats->paths = total;
@@ -1510,12 +1815,28 @@ int render_impl(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_des
         stats->waves = n_waves;
         stats->bounces_max = uint32_t(bounces);
         for (int k = 0; k < TR_K_COUNT; ++k) {
-            stats->kernel_ms[k] = kernel_ms[k];
-            stats->kernel_launches[k] = launches[k];
+            stats->kernel_ms[k] = L.kernel_ms[k];
+            stats->kernel_launches[k] = L.launches[k];
         }
         stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     }
     return TR_OK;
+}
+
+int render_impl(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* d, tr_guide guide,
+                tr_render_stats* stats)
+{
+    auto t_start = std::chrono::steady_clock::now();
+    std::vector<HostJob> jobs(1);
+    int rc = make_job(s, cam, f, d, &jobs[0], guide ? "tr_render_transient_guided" : "tr_render_transient");
+    if (rc) return rc;
+    LaunchCfg c = s->tune;
+    if (d->wave_paths) c.wave = d->wave_paths;
+    c.wide = (d->flags & TR_RENDER_BINARY_BVH) == 0;
+    c.graph = (d->flags & TR_RENDER_GRAPH) != 0;
+    c.prof = (d->flags & TR_RENDER_PROFILE) != 0;
+    c.mode = guide ? kShadeGuided : ((d->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef);
+    return run_jobs(s->stream, s->wave, jobs, c, guide, s->compute_units, s->ev, &s->ev_made, stats, t_start);
 }
 
 }  // namespace
@@ -1532,6 +1853,193 @@ int tr_render_transient_guided(tr_scene s, const tr_camera* cam, tr_film f, cons
 {
     if (!g) return fail(TR_ERR_INVALID, "tr_render_transient_guided: null guide");
     return render_impl(s, cam, f, d, g, stats);
+}
+
+}  // extern "C"
+
+// =============================================================================================
+// batched renders and the launch-shape autotuner
+
+struct tr_batch_s {
+    struct Entry {
+        tr_scene s;
+        tr_camera cam;
+        tr_film f;
+        tr_render_desc d;
+    };
+    std::vector<Entry> entries;
+    hipStream_t stream = nullptr;
+    WaveBuffers wave;
+    hipEvent_t ev[2 * 256] = {};
+    bool ev_made = false;
+    int compute_units = 256;
+    bool tuned = false;
+    LaunchCfg tune;
+
+    ~tr_batch_s()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        wave.release();
+        if (ev_made)
+            for (auto& e : ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+LaunchCfg launch_cfg(const tr_batch_s* b, const tr_launch_desc* L)
+{
+    LaunchCfg c = b->tuned ? b->tune : LaunchCfg{};
+    if (L) {
+        if (L->wave_paths) c.wave = L->wave_paths;
+        if (L->trace_per_cu) c.trace_per_cu = L->trace_per_cu;
+        if (L->shade_per_cu) c.shade_per_cu = L->shade_per_cu;
+        c.wide = (L->flags & TR_RENDER_BINARY_BVH) == 0;
+        c.graph = (L->flags & TR_RENDER_GRAPH) != 0;
+        c.prof = (L->flags & TR_RENDER_PROFILE) != 0;
+        c.mode = (L->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef;
+    }
+    return c;
+}
+
+int batch_jobs(tr_batch b, std::vector<HostJob>* jobs, int sqrt_spp_override)
+{
+    jobs->resize(b->entries.size());
+    for (size_t k = 0; k < b->entries.size(); ++k) {
+        const auto& e = b->entries[k];
+        tr_render_desc d = e.d;
+        if (sqrt_spp_override > 0) d.sqrt_spp = sqrt_spp_override;
+        int rc = make_job(e.s, &e.cam, e.f, &d, &(*jobs)[k], "tr_batch_render");
+        if (rc) return rc;
+    }
+    return TR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tr_batch_create(tr_batch* out)
+{
+    if (!out) return fail(TR_ERR_INVALID, "tr_batch_create: null output");
+    *out = nullptr;
+    int rc = check_device();
+    if (rc) return rc;
+    tr_batch b = new (std::nothrow) tr_batch_s();
+    if (!b) return fail(TR_ERR_NOMEM, "tr_batch_create: host allocation");
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(TR_ERR_DEVICE, "tr_batch_create: hipStreamCreate failed");
+    }
+    b->compute_units = tri::compute_units();
+    *out = b;
+    return TR_OK;
+}
+
+int tr_batch_destroy(tr_batch b)
+{
+    delete b;
+    return TR_OK;
+}
+
+int tr_batch_add(tr_batch b, tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* d, uint32_t* job)
+{
+    if (!b || !s || !cam || !f || !d) return fail(TR_ERR_INVALID, "tr_batch_add: null argument");
+    for (const auto& e : b->entries)
+        if (e.f == f) return fail(TR_ERR_INVALID, "tr_batch_add: every job needs its own film");
+    if (b->entries.size() >= 65535u) return fail(TR_ERR_INVALID, "tr_batch_add: too many jobs");
+    HostJob probe;
+    int rc = make_job(s, cam, f, d, &probe, "tr_batch_add");
+    if (rc) return rc;
+    b->entries.push_back({s, *cam, f, *d});
+    if (job) *job = uint32_t(b->entries.size() - 1);
+    return TR_OK;
+}
+
+int tr_batch_clear(tr_batch b)
+{
+    if (!b) return fail(TR_ERR_INVALID, "tr_batch_clear: null batch");
+    b->entries.clear();
+    return TR_OK;
+}
+
+int tr_batch_render(tr_batch b, const tr_launch_desc* launch, tr_render_stats* stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    if (!b) return fail(TR_ERR_INVALID, "tr_batch_render: null batch");
+    if (b->entries.empty()) return fail(TR_ERR_INVALID, "tr_batch_render: no job");
+    std::vector<HostJob> jobs;
+    int rc = batch_jobs(b, &jobs, 0);
+    if (rc) return rc;
+    return run_jobs(b->stream, b->wave, jobs, launch_cfg(b, launch), nullptr, b->compute_units, b->ev, &b->ev_made,
+                    stats, t0);
+}
+
+int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
+{
+    if (!b) return fail(TR_ERR_INVALID, "tr_autotune: null batch");
+    if (b->entries.empty()) return fail(TR_ERR_INVALID, "tr_autotune: no job");
+    std::vector<HostJob> jobs;
+    int rc = batch_jobs(b, &jobs, 1);
+    if (rc) return rc;
+    uint64_t total = 0;
+    for (auto& j : jobs) total += j.paths;
+    LaunchCfg base = b->tuned ? b->tune : LaunchCfg{};
+    // one trial: the jobs at one camera sample per pixel, best of two runs
+    auto trial = [&](const LaunchCfg& c, double* ms) -> int {
+        double m = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+            tr_render_stats st;
+            auto t0 = std::chrono::steady_clock::now();
+            int r = run_jobs(b->stream, b->wave, jobs, c, nullptr, b->compute_units, b->ev, &b->ev_made, &st, t0);
+            if (r) return r;
+            m = std::min(m, st.render_ms);
+        }
+        *ms = m;
+        return TR_OK;
+    };
+    LaunchCfg bestc = base;
+    double best_ms = 1e300;
+    // stage 1: wavefront size and graph replay; stage 2: persistent grid shapes
+    std::vector<uint32_t> waves;
+    for (uint32_t w : {1u << 20, 1u << 22, 1u << 24, 1u << 25})
+        if (w <= std::max<uint64_t>(total, 1u << 20)) waves.push_back(w);
+    for (uint32_t w : waves)
+        for (bool g : {false, true}) {
+            LaunchCfg c = base;
+            c.wave = w;
+            c.graph = g;
+            c.prof = false;
+            double ms;
+            if ((rc = trial(c, &ms))) return rc;
+            if (ms < best_ms) best_ms = ms, bestc = c;
+        }
+    for (uint32_t tpc : {8u, 16u, 32u})
+        for (uint32_t spc : {1u, 2u, 4u}) {
+            if (tpc == bestc.trace_per_cu && spc == bestc.shade_per_cu) continue;
+            LaunchCfg c = bestc;
+            c.trace_per_cu = tpc;
+            c.shade_per_cu = spc;
+            double ms;
+            if ((rc = trial(c, &ms))) return rc;
+            if (ms < best_ms) best_ms = ms, bestc = c;
+        }
+    // the trials accumulated into the films
+    for (auto& e : b->entries)
+        if ((rc = tr_film_clear(e.f))) return rc;
+    b->tune = bestc;
+    b->tuned = true;
+    if (b->entries.size() == 1) b->entries[0].s->tune = bestc;
+    if (best) {
+        best->wave_paths = bestc.wave;
+        best->flags = (bestc.wide ? 0u : TR_RENDER_BINARY_BVH) | (bestc.graph ? TR_RENDER_GRAPH : 0u) |
+                      (bestc.mode == kShadeOnb ? TR_RENDER_ORTHONORMAL_FRAME : 0u);
+        best->trace_per_cu = bestc.trace_per_cu;
+        best->shade_per_cu = bestc.shade_per_cu;
+    }
+    if (ms_best) *ms_best = best_ms;
+    return TR_OK;
 }
 
 }  // extern "C"

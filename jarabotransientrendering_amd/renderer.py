@@ -257,25 +257,16 @@ class StreakCameraFilm:
         nat.check(nat.lib().tr_film_read_sums(self.handle, nat.as_ptr(s), nat.as_ptr(w)))
         return s, w
 
-    def write(self, name: str, extension: str = "raw") -> List[str]:
-        """Per-scanline streak files like StreakCameraFilm::write (StreakCameraFilm.h:233-242, 399-437):
-        `<name>_%04d.raw`, each a (W x T x 3) float32 image (transposed when W < 8 as upstream)."""
-        if extension != "raw":
-            raise ValueError("only .raw output is provided")
-        data = self.read()
-        rows = [self.streak] if self.streak >= 0 else range(self.height)
-        paths = []
+    def write(self, name: str, extension: str = "hdr") -> List[str]:
+        """StreakCameraFilm::write + Film::write (StreakCameraFilm.h:233-242, 357-437; Film.h:358-471):
+        `<name>_%04d.<extension>` per scanline (an image T wide and W high, transposed when T < 8),
+        `extension` "hdr" (Radiance RGBE) or "raw" (FP32), then the steady image `<name>.hdr`."""
         d = os.path.dirname(name)
         if d:
             os.makedirs(d, exist_ok=True)
-        for i, y in enumerate(rows):
-            img = data[i]  # (W, T, 3): Image(width=T, height=W)
-            if self.time_bins < 8:
-                img = np.ascontiguousarray(img.transpose(1, 0, 2))
-            p = f"{name}_{y:04d}.raw"
-            img.astype(np.float32).tofile(p)
-            paths.append(p)
-        return paths
+        nat.check(nat.lib().tr_film_write(self.handle, name.encode(), extension.encode()))
+        rows = [self.streak] if self.streak >= 0 else range(self.height)
+        return [f"{name}_{y:04d}.{extension}" for y in rows] + [f"{name}.hdr"]
 
 
 class BidirectionalPathTracing:
@@ -381,6 +372,7 @@ class TransientRenderer:
         self.profile = False
         self.binary_bvh = False   # traverse the binary BVH instead of the 4-wide one
         self.orthonormal_frame = False  # Lambertian sampling frame without the reference's quirk
+        self.graph = False              # replay each wavefront from a captured hipGraph
         self.last_stats: Optional[dict] = None
 
     def render(self, name: Optional[str], world: World, integrator: Optional[BidirectionalPathTracing],
@@ -396,7 +388,8 @@ class TransientRenderer:
             raise ValueError("sampler and film sizes differ")
         integrator.preprocess()
         flags = ((nat.TR_RENDER_PROFILE if self.profile else 0) | (nat.TR_RENDER_BINARY_BVH if self.binary_bvh else 0)
-                 | (nat.TR_RENDER_ORTHONORMAL_FRAME if self.orthonormal_frame else 0))
+                 | (nat.TR_RENDER_ORTHONORMAL_FRAME if self.orthonormal_frame else 0)
+                 | (nat.TR_RENDER_GRAPH if self.graph else 0))
         d = nat.RenderDesc(sampler.sqrt_spp, integrator.incoming_samples, integrator.max_path_size, 0, int(seed),
                            int(self.wave_paths), flags)
         st = nat.RenderStats()
@@ -414,6 +407,55 @@ class TransientRenderer:
 
 
 # ---------------------------------------------------------------------------------------------
+class Batch:
+    """Many render jobs in one launch sequence (tr_batch_*; no counterpart in the reference).
+    Each job accumulates into its own film exactly as TransientRenderer.render would."""
+
+    def __init__(self):
+        self.handle = ctypes.c_void_p()
+        nat.check(nat.lib().tr_batch_create(ctypes.byref(self.handle)))
+        self._keep = []  # scenes and films must outlive the batch's renders
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                nat.lib().tr_batch_destroy(h)
+            except Exception:
+                pass
+
+    def add(self, world: World, integrator: BidirectionalPathTracing, camera: PinholeCamera,
+            film: StreakCameraFilm, sampler: StratifiedSampler, seed: int = 0) -> int:
+        if not world.frozen:
+            world.freeze()
+        if sampler.scanline >= 0 and film.streak != sampler.scanline:
+            film.set_streak(sampler.scanline)
+        d = nat.RenderDesc(sampler.sqrt_spp, integrator.incoming_samples, integrator.max_path_size, 0, int(seed), 0, 0)
+        cam = camera.desc()
+        job = ctypes.c_uint32()
+        nat.check(nat.lib().tr_batch_add(self.handle, world.handle, ctypes.byref(cam), film.handle, ctypes.byref(d),
+                                         ctypes.byref(job)))
+        self._keep.append((world, film))
+        return int(job.value)
+
+    @staticmethod
+    def _launch(wave_paths=0, flags=0, trace_per_cu=0, shade_per_cu=0):
+        return nat.LaunchDesc(int(wave_paths), int(flags), int(trace_per_cu), int(shade_per_cu))
+
+    def render(self, launch: Optional[dict] = None) -> dict:
+        st = nat.RenderStats()
+        L = self._launch(**launch) if launch is not None else None
+        nat.check(nat.lib().tr_batch_render(self.handle, ctypes.byref(L) if L is not None else None,
+                                            ctypes.byref(st)))
+        return st.as_dict()
+
+    def autotune(self):
+        best = nat.LaunchDesc()
+        ms = ctypes.c_double()
+        nat.check(nat.lib().tr_autotune(self.handle, ctypes.byref(best), ctypes.byref(ms)))
+        return best.as_dict(), float(ms.value)
+
+
 def setup_scene(scene):
     """Build (world, camera, sampler, film, integrator) for a `scenes.Scene`, the way
     bunnykiller.cpp wires them from command-line flags."""
@@ -448,7 +490,8 @@ def setup_scene(scene):
 
 
 def render_scene(scene, seed: Optional[int] = None, wave_paths: int = 0, profile: bool = False,
-                 binary_bvh: bool = False, orthonormal_frame: bool = False, guide: Optional[PathGuide] = None):
+                 binary_bvh: bool = False, orthonormal_frame: bool = False, guide: Optional[PathGuide] = None,
+                 graph: bool = False):
     """Render a `scenes.Scene`; returns (normalised film [rows][W][T][3], stats, film object)."""
     w, cam, sampler, film, integ = setup_scene(scene)
     eng = TransientRenderer()
@@ -456,6 +499,7 @@ def render_scene(scene, seed: Optional[int] = None, wave_paths: int = 0, profile
     eng.profile = profile
     eng.binary_bvh = binary_bvh
     eng.orthonormal_frame = orthonormal_frame
+    eng.graph = graph
     st = eng.render(None, w, integ, cam, film, sampler, seed=scene.render.seed if seed is None else seed,
                     guide=guide)
     return film.read(), st, film

@@ -274,3 +274,30 @@ def media_box(width=512, height=512, scanline=256, time_bins=1024, exposure=0.01
     s.film.scanline = scanline
     s.name = "media_box"
     return s
+
+
+def batch_scene_params(seed: int):
+    """Draws of one batched-workload job, log-uniform from the job's seed: a triangle target in
+    [1e3, 5e5] (met by the nearest icosphere subdivision level, 20 * 4^level triangles), a square
+    film of 32..512 px, 128..2048 time bins and 64..1024 samples per pixel."""
+    rng = np.random.default_rng(seed)
+    target = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+    level = min(range(3, 8), key=lambda l: abs(np.log(20 * 4 ** l) - np.log(target)))
+    size = int(round(np.exp(rng.uniform(np.log(32), np.log(512)))))
+    bins = int(round(np.exp(rng.uniform(np.log(128), np.log(2048)))))
+    spp = float(np.exp(rng.uniform(np.log(64), np.log(1024))))
+    return {"triangle_target": target, "level": level, "size": size, "time_bins": bins,
+            "sqrt_spp": max(1, int(round(spp ** 0.5)))}
+
+
+def batch_scenes(n: int = 64, first_seed: int = 0) -> List[Scene]:
+    """The batched workload: n uneven icosphere-in-box scenes, seeds first_seed .. first_seed + n - 1."""
+    out = []
+    for k in range(first_seed, first_seed + n):
+        p = batch_scene_params(k)
+        s = icosphere_box(level=p["level"], width=p["size"], height=p["size"], time_bins=p["time_bins"],
+                          exposure=0.01 * 1024 / p["time_bins"], sqrt_spp=p["sqrt_spp"], n_incoming=1,
+                          max_bounces=8, seed=k)
+        s.name = f"batch{k}_ico{p['level']}"
+        out.append(s)
+    return out

@@ -273,8 +273,28 @@ static int run_rng(int argc, char** argv)
     return 0;
 }
 
+// save <in.f32> <W> <H> <out.hdr|out.raw>: an RGB float image through Imaging::save (ImageIO.h)
+static int run_save(int argc, char** argv)
+{
+    (void)argc;
+    unsigned W = unsigned(atoi(argv[3])), H = unsigned(atoi(argv[4]));
+    std::vector<float> px(size_t(W) * H * 3);
+    FILE* f = fopen(argv[2], "rb");
+    if (!f || fread(px.data(), sizeof(float), px.size(), f) != px.size()) {
+        fprintf(stderr, "cannot read %s\n", argv[2]);
+        return 1;
+    }
+    fclose(f);
+    std::unique_ptr<Real[]> buf(new Real[px.size()]);
+    std::copy(px.begin(), px.end(), buf.get());
+    Imaging::Image<Real> img(std::move(buf), W, H, 3);
+    Imaging::save(img, argv[5]);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
+    if (argc >= 6 && !strcmp(argv[1], "save")) return run_save(argc, argv);
     if (argc >= 4 && !strcmp(argv[1], "rng")) return run_rng(argc, argv);
     if (argc >= 3 && !strcmp(argv[1], "render")) return run_render(parse_scene(argv[2]), false);
     if (argc >= 3 && !strcmp(argv[1], "render_cs")) return run_render(parse_scene(argv[2]), true);

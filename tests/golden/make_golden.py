@@ -14,6 +14,8 @@ scenes below and records:
   stats_<name>.npz               per-bin mean / standard deviation over K seeds
   isect_<name>.npz               closest-hit triangle ids and distances of seeded
                                  rays from the reference's Triangle::intersect
+  image_<name>.npz               an RGB float image and the Radiance .hdr bytes the
+                                 reference's Imaging::save wrote for it
   manifest.json                  scene parameters, harness mode, seeds, and a
                                  SHA-256 of each scene's FP32 triangles
 
@@ -107,8 +109,27 @@ def isect_cases():
     return out
 
 
-def main():
+def image_cases():
+    """Float RGB images for the RGBE writer: narrow rows (stored without run-length encoding),
+    long rows with runs, literal spans, zeros, a huge dynamic range and negative values."""
+    rng = np.random.default_rng(5)
+    out = []
+    out.append(("narrow", rng.random((3, 5, 3), dtype=np.float32) * 4))
+    img = np.zeros((4, 300, 3), np.float32)
+    img[0] = rng.random((300, 3)) * 10.0 ** rng.uniform(-6, 4, (300, 1))
+    img[1, 50:250] = [0.25, 0.5, 1.0]              # long runs
+    img[2, ::2] = 1e-3                             # alternating values: literal spans
+    img[3] = rng.normal(size=(300, 3))             # negatives are clamped to 0
+    out.append(("runs", img))
+    out.append(("streak_like", (rng.random((6, 512, 3)) ** 8 * 1e3).astype(np.float32)))
+    return out
+
+
+def main(only=None):
     subprocess.check_call(["sh", os.path.join(ROOT, "oracle", "build_ref.sh")])
+    if only == "images":
+        write_images()
+        return
     manifest = {"generator": "tests/golden/make_golden.py", "harness": "oracle/ref_harness.cpp",
                 "host": platform.processor() or platform.machine(), "films": {}, "stats": {}, "isect": {}}
     # reference RNG: published pcg32-demo seeding (42, 54) and the renderer's seed(0)
@@ -146,9 +167,23 @@ def main():
                                 t=hits[:, 1].view(np.float32))
             manifest["isect"][tag] = {"n_rays": int(rays.shape[0]), **describe(s)}
             print(tag, "isect", int((hits[:, 0] >= 0).sum()), "hits")
+    write_images()
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, default=float)
 
 
+def write_images():
+    with tempfile.TemporaryDirectory() as tmp:
+        for tag, img in image_cases():
+            H, W, _ = img.shape
+            src = os.path.join(tmp, tag + ".f32")
+            dst = os.path.join(tmp, tag + ".hdr")
+            np.ascontiguousarray(img, dtype=np.float32).tofile(src)
+            subprocess.check_call([HARNESS, "save", src, str(W), str(H), dst])
+            data = np.fromfile(dst, dtype=np.uint8)
+            np.savez_compressed(os.path.join(HERE, f"image_{tag}.npz"), image=img, hdr=data)
+            print(tag, img.shape, data.size, "bytes")
+
+
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else None)
