@@ -118,6 +118,9 @@ typedef struct {
     float offset;         /* -film-offset (StreakCameraFilm::set_offset) */
     int32_t scanline;     /* -film-scanline y, or -1 for the whole film */
     int32_t camera_unwarp;/* -camera-unwarp */
+    /* -multibounce-streak N: MultibounceStreakFilm (src/Film/MultibounceStreakCameraFilm.h:67-171)
+     * with N bounce films (a sample of bounce b >= N goes to the last one); 0 = one streak film */
+    int32_t bounce_layers;
 } tr_film_desc;
 
 /* StreakCameraFilm::StreakCameraFilm (src/Film/StreakCameraFilm.h:91-121). Device buffers:
@@ -129,9 +132,13 @@ int tr_film_destroy(tr_film film);
 int tr_film_clear(tr_film film);
 /* Normalised streak data, StreakCameraFilm::average_and_save_slice (StreakCameraFilm.h:357-397)
  * with every pixel divided by its own weight (0/0 -> 0): out[row][x][t][c], rows x W x T x 3.
- * steady (optional, may be NULL): the time-integrated image out[row][x][c] (Film::draw_pixel). */
+ * steady (optional, may be NULL): the time-integrated image out[row][x][c] (Film::draw_pixel).
+ * Multibounce films: out[layer][row][x][t][c], each bounce film divided by its own weights (the
+ * camera samples that sent it samples, MultibounceStreakCameraFilm.h:137-157); steady holds every
+ * sample, inside the time window or not (Film::add_samples, Film.h:263-283). */
 int tr_film_read(tr_film film, float* out, float* steady);
-/* Un-normalised compensated sums (sum + compensation) as double, and per-pixel weights. */
+/* Un-normalised compensated sums (sum + compensation) as double, and per-pixel weights
+ * ([layer][row][x] for multibounce films). */
 int tr_film_read_sums(tr_film film, double* sums, double* weights);
 /* Device pointers of the compensated-sum buffers (read-only use by the caller). */
 int tr_film_device_buffers(tr_film film, float** d_sum, float** d_comp);

@@ -179,7 +179,7 @@ class StreakCameraFilm {
 public:
     StreakCameraFilm(unsigned w, unsigned h, unsigned t, Real exposure, bool camera_unwarp = false)
     {
-        d_ = tr_film_desc{int32_t(w), int32_t(h), int32_t(t), exposure, 0.f, -1, camera_unwarp ? 1 : 0};
+        d_ = tr_film_desc{int32_t(w), int32_t(h), int32_t(t), exposure, 0.f, -1, camera_unwarp ? 1 : 0, 0};
     }
     ~StreakCameraFilm()
     {
@@ -208,7 +208,8 @@ public:
     // normalised streak data: [row][x][t][rgb]
     std::vector<float> data()
     {
-        std::vector<float> out(size_t(rows()) * d_.width * d_.time_bins * 3);
+        std::vector<float> out(size_t(d_.bounce_layers > 0 ? d_.bounce_layers : 1) * size_t(rows()) * d_.width *
+                               d_.time_bins * 3);
         check(tr_film_read(handle(), out.data(), nullptr), "StreakCameraFilm::data");
         return out;
     }
@@ -222,7 +223,7 @@ public:
         check(tr_film_write(handle(), base.c_str(), ext_.c_str()), "StreakCameraFilm::write");
     }
 
-private:
+protected:
     void unallocated() const
     {
         if (h_) throw std::runtime_error("StreakCameraFilm: geometry is fixed once rendering started");
@@ -230,6 +231,20 @@ private:
     tr_film_desc d_{};
     tr_film h_ = nullptr;
     std::string name_, ext_ = "raw";
+};
+
+// MultibounceStreakFilm (src/Film/MultibounceStreakCameraFilm.h:67-171): one streak film per
+// bounce ("<name>_bXX_%04d.<ext>", XX = bounce + 1; bounces >= nb_bounces go to the last one)
+class MultibounceStreakFilm : public StreakCameraFilm {
+public:
+    MultibounceStreakFilm(unsigned w, unsigned h, unsigned t, Real exposure, unsigned nb_bounces,
+                          bool camera_unwarp = false)
+        : StreakCameraFilm(w, h, t, exposure, camera_unwarp)
+    {
+        if (nb_bounces == 0 || nb_bounces > 32) throw std::runtime_error("MultibounceStreakFilm: 1..32 bounces");
+        d_.bounce_layers = int32_t(nb_bounces);
+    }
+    unsigned nb_bounces() const { return unsigned(d_.bounce_layers); }
 };
 
 // BidirectionalPathTracing in path-tracing-only SteadyState mode (BidirectionalPathTracing.h:122-217)

@@ -61,7 +61,7 @@ class Hit(ctypes.Structure):
 class FilmDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("time_bins", ctypes.c_int32),
                 ("exposure", ctypes.c_float), ("offset", ctypes.c_float), ("scanline", ctypes.c_int32),
-                ("camera_unwarp", ctypes.c_int32)]
+                ("camera_unwarp", ctypes.c_int32), ("bounce_layers", ctypes.c_int32)]
 
 
 class CameraDesc(ctypes.Structure):

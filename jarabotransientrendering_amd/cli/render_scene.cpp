@@ -11,11 +11,13 @@
 //   point x y z r g b
 //   rect cx cy cz tx ty tz size_x size_y intensity
 //   medium sigma_a(rgb) sigma_s(rgb)
+//   multibounce nb_bounces
 //   out name_prefix
 // Output: <prefix>_%04d.raw streak slices, <prefix>.log, and one JSON line of render statistics.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -42,6 +44,7 @@ int main(int argc, char** argv)
         std::vector<tr::RectangularAreaLightSource> rects;
         bool has_medium = false;
         tr::Vector3 sa{}, ss{};
+        int multibounce = 0;
         std::string out = "film";
         std::string line;
         // consecutive triangles of one material form one mesh, as a -name-mesh per material
@@ -84,6 +87,8 @@ int main(int argc, char** argv)
             } else if (key == "medium") {
                 ls >> sa[0] >> sa[1] >> sa[2] >> ss[0] >> ss[1] >> ss[2];
                 has_medium = true;
+            } else if (key == "multibounce") {
+                ls >> multibounce;
             } else if (key == "out") {
                 ls >> out;
             } else {
@@ -107,7 +112,13 @@ int main(int argc, char** argv)
             {float(cam[6]), float(cam[7]), float(cam[8])}, float(cam[9]));
         w.freeze();
         tr::StratifiedSampler sampler(W, H, sqrt_spp, true);
-        tr::StreakCameraFilm film(unsigned(W), unsigned(H), unsigned(T), exposure, unwarp != 0);
+        std::unique_ptr<tr::StreakCameraFilm> film_ptr;
+        if (multibounce > 0)
+            film_ptr.reset(new tr::MultibounceStreakFilm(unsigned(W), unsigned(H), unsigned(T), exposure,
+                                                         unsigned(multibounce), unwarp != 0));
+        else
+            film_ptr.reset(new tr::StreakCameraFilm(unsigned(W), unsigned(H), unsigned(T), exposure, unwarp != 0));
+        tr::StreakCameraFilm& film = *film_ptr;
         film.set_name(out, "raw");
         if (offset > 0.0f) film.set_offset(offset);
         if (scanline > -1) {

@@ -93,6 +93,12 @@ struct RenderDev {
     float3 cpos, nv, uv, vv;
     float fd, xw2, yw2, ratio, inv_spp;
     uint64_t seed;
+    // MultibounceStreakFilm: `layers` bounce films of layer_stride bins each (0 = plain film)
+    int32_t layers;
+    uint32_t layer_stride;
+    float* steady;             // [rows][W][3] every sample, any time (multibounce films)
+    float* layer_w;            // [layers][rows][W] camera samples with samples of each layer
+    uint32_t* cs_bits;         // per camera sample of the wavefront: layers that received samples
 };
 
 struct WaveDev {
@@ -479,7 +485,23 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                         float sensor = (cam_time - R.offset) / R.exposure;
                         bool finite = isfinite(val.x) && isfinite(val.y) && isfinite(val.z);
                         bool nonzero = val.x != 0.0f || val.y != 0.0f || val.z != 0.0f;
-                        if (sensor >= 0.0f && sensor < float(R.T) && finite && nonzero) {
+                        const bool in_window = sensor >= 0.0f && sensor < float(R.T);
+                        if (!BATCH && !GUIDED && R.layers > 0) {
+                            // multibounce film: every connected sample counts for its layer's
+                            // weight and the steady image, in the time window or not
+                            // (MultibounceStreakCameraFilm.h:137-157, Film.h:263-283)
+                            const uint32_t layer = min(level, uint32_t(R.layers - 1));
+                            uint32_t fidx = 0xFFFFFFFFu;
+                            if (in_window)
+                                fidx = layer * R.layer_stride + (extra.z >> 8) * uint32_t(R.T) +
+                                       min(uint32_t(floorf(sensor)), uint32_t(R.T - 1));
+                            if (!finite) val = mk(0.0f, 0.0f, 0.0f);  // TransientRenderer.h:167-171
+                            emit_shadow = true;
+                            sh_o = make_float4(lp.x, lp.y, lp.z, ldist);
+                            sh_d = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(fidx));
+                            sh_c = make_float4(val.x, val.y, val.z, __uint_as_float(extra.z >> 8));
+                            sh_tag = ((slot / uint32_t(R.n_inc)) << 5) | layer;
+                        } else if (in_window && finite && nonzero) {
                             uint32_t bin = min(uint32_t(floorf(sensor)), uint32_t(R.T - 1));
                             uint32_t fidx = (extra.z >> 8) * uint32_t(R.T) + bin;
                             emit_shadow = true;
@@ -612,7 +634,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
             Wv.sh_o[sh_pos] = sh_o;
             Wv.sh_d[sh_pos] = sh_d;
             Wv.sh_c[sh_pos] = sh_c;
-            if constexpr (GUIDED) Wv.sh_tag[sh_pos] = sh_tag;
+            if (GUIDED || (!BATCH && R.layers > 0)) Wv.sh_tag[sh_pos] = sh_tag;
             if constexpr (BATCH) Wv.sh_job[sh_pos] = jid;
         }
         uint32_t q_pos = block_append(Wv.counters + (cur ^ 1), cont ? 1u : 0u, scan);
@@ -727,6 +749,48 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
             }
         }
         __syncthreads();
+    }
+}
+
+// multibounce films: visible connections mark their layer for the camera sample, add to the
+// steady image, and (inside the time window) to their layer's pass film
+__global__ void __launch_bounds__(kBlock) film_mb_kernel(RenderDev R, WaveDev Wv)
+{
+    const uint32_t count = Wv.counters[C_SHADOW];
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count; r += gridDim.x * blockDim.x) {
+        if (!Wv.sh_vis[r]) continue;
+        const float4 c = Wv.sh_c[r];
+        const uint32_t tag = Wv.sh_tag[r], pix = __float_as_uint(c.w), fidx = __float_as_uint(Wv.sh_d[r].w);
+        atomicOr(R.cs_bits + (tag >> 5), 1u << (tag & 31u));
+        if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
+            unsafeAtomicAdd(R.steady + size_t(pix) * 3 + 0, c.x);
+            unsafeAtomicAdd(R.steady + size_t(pix) * 3 + 1, c.y);
+            unsafeAtomicAdd(R.steady + size_t(pix) * 3 + 2, c.z);
+            if (fidx != 0xFFFFFFFFu) {
+                float* dst = R.film + size_t(fidx) * 3;
+                unsafeAtomicAdd(dst + 0, c.x);
+                unsafeAtomicAdd(dst + 1, c.y);
+                unsafeAtomicAdd(dst + 2, c.z);
+            }
+        }
+    }
+}
+
+// multibounce films, after a wavefront: each camera sample adds 1 to the weight of every layer it
+// sent samples to (a bounce film only sees those camera samples), then its marks are cleared
+__global__ void __launch_bounds__(kBlock) mb_weight_kernel(RenderDev R, WaveDev Wv)
+{
+    const WaveCtl ctl = *Wv.ctl;
+    const uint32_t n_cs = ctl.n / uint32_t(R.n_inc);
+    const uint32_t cs0 = uint32_t(ctl.base / uint64_t(R.n_inc));
+    const uint32_t pixels = R.layer_stride / uint32_t(R.T);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_cs; i += gridDim.x * blockDim.x) {
+        uint32_t bits = R.cs_bits[i];
+        if (!bits) continue;
+        R.cs_bits[i] = 0u;
+        const uint32_t pix = (cs0 + i) / uint32_t(R.spp2);
+        for (int b = 0; b < R.layers; ++b)
+            if (bits & (1u << b)) atomicAdd(R.layer_w + size_t(b) * pixels + pix, 1.0f);
     }
 }
 
@@ -909,15 +973,27 @@ struct WaveBuffers {
     WaveCtl* ctl = nullptr;
     JobDev* d_jobs = nullptr;
     uint32_t jobs_cap = 0;
+    uint32_t* cs_bits = nullptr;  // multibounce films: layer marks per camera sample of a wavefront
+    uint32_t cs_cap = 0;
     uint32_t* counters = nullptr;
 
     void release()
     {
         dev_free(ray_o); dev_free(ray_d); dev_free(thr); dev_free(misc); dev_free(extra); dev_free(hit);
         dev_free(q0); dev_free(q1); dev_free(sh_o); dev_free(sh_d); dev_free(sh_c); dev_free(sh_vis); dev_free(counters);
-        dev_free(sh_tag); dev_free(job); dev_free(sh_job); dev_free(ctl); dev_free(d_jobs);
+        dev_free(sh_tag); dev_free(job); dev_free(sh_job); dev_free(ctl); dev_free(d_jobs); dev_free(cs_bits);
         cap = 0;
         jobs_cap = 0;
+        cs_cap = 0;
+    }
+    int ensure_cs(uint32_t n)
+    {
+        if (n <= cs_cap) return TR_OK;
+        dev_free(cs_bits);
+        int rc = dev_alloc(&cs_bits, n);
+        if (rc) return rc;
+        cs_cap = n;
+        return TR_OK;
     }
     int ensure_batch(uint32_t n_jobs)
     {
@@ -1030,8 +1106,10 @@ struct tr_scene_s {
 struct tr_film_s {
     tr_film_desc desc{};
     int rows = 0;
-    size_t n_elems = 0;  // rows*W*T*3
+    int layers = 0;      // multibounce bounce films (0: one streak film)
+    size_t n_elems = 0;  // max(layers, 1) * rows*W*T*3
     float *d_pass = nullptr, *d_sum = nullptr, *d_comp = nullptr;
+    float *d_layer_w = nullptr, *d_steady = nullptr;  // multibounce films only
     std::vector<double> weight;  // rows*W
     hipStream_t stream = nullptr;
 };
@@ -1321,24 +1399,33 @@ int tr_film_create(tr_film* out, const tr_film_desc* d)
     if (d->width <= 0 || d->height <= 0 || d->time_bins <= 0 || !(d->exposure > 0.f) ||
         d->scanline >= d->height || d->scanline < -1)
         return fail(TR_ERR_INVALID, "tr_film_create: bad film geometry");
+    if (d->bounce_layers < 0 || d->bounce_layers > 32)
+        return fail(TR_ERR_INVALID, "tr_film_create: bounce_layers must be in [0, 32]");
     int rc = check_device();
     if (rc) return rc;
     int rows = d->scanline >= 0 ? 1 : d->height;
-    size_t n = size_t(rows) * size_t(d->width) * size_t(d->time_bins) * 3;
+    const int layers = d->bounce_layers;
+    size_t n = size_t(std::max(layers, 1)) * size_t(rows) * size_t(d->width) * size_t(d->time_bins) * 3;
     if (n / 3 >= 0xFFFFFFFFull) return fail(TR_ERR_INVALID, "tr_film_create: film larger than 2^32 bins");
     tr_film f = new (std::nothrow) tr_film_s();
     if (!f) return fail(TR_ERR_NOMEM, "tr_film_create: host allocation");
     f->desc = *d;
     f->rows = rows;
+    f->layers = layers;
     f->n_elems = n;
     size_t n_alloc = (n + 3) / 4 * 4;  // merge kernel works on float4
-    if ((rc = dev_alloc(&f->d_pass, n_alloc)) || (rc = dev_alloc(&f->d_sum, n_alloc)) || (rc = dev_alloc(&f->d_comp, n_alloc))) {
-        dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp);
+    auto release = [&]() {
+        dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp); dev_free(f->d_layer_w); dev_free(f->d_steady);
+    };
+    if ((rc = dev_alloc(&f->d_pass, n_alloc)) || (rc = dev_alloc(&f->d_sum, n_alloc)) || (rc = dev_alloc(&f->d_comp, n_alloc)) ||
+        (layers && ((rc = dev_alloc(&f->d_layer_w, size_t(layers) * rows * size_t(d->width))) ||
+                    (rc = dev_alloc(&f->d_steady, size_t(rows) * size_t(d->width) * 3))))) {
+        release();
         delete f;
         return rc;
     }
     if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess) {
-        dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp);
+        release();
         delete f;
         return fail(TR_ERR_DEVICE, "tr_film_create: hipStreamCreate failed");
     }
@@ -1356,7 +1443,7 @@ int tr_film_destroy(tr_film f)
 {
     if (!f) return TR_OK;
     if (f->stream) (void)hipStreamSynchronize(f->stream);
-    dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp);
+    dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp); dev_free(f->d_layer_w); dev_free(f->d_steady);
     if (f->stream) (void)hipStreamDestroy(f->stream);
     delete f;
     return TR_OK;
@@ -1369,6 +1456,11 @@ int tr_film_clear(tr_film f)
     HIP_TRY(hipMemsetAsync(f->d_pass, 0, bytes, f->stream));
     HIP_TRY(hipMemsetAsync(f->d_sum, 0, bytes, f->stream));
     HIP_TRY(hipMemsetAsync(f->d_comp, 0, bytes, f->stream));
+    if (f->layers) {
+        const size_t px = size_t(f->rows) * size_t(f->desc.width);
+        HIP_TRY(hipMemsetAsync(f->d_layer_w, 0, size_t(f->layers) * px * sizeof(float), f->stream));
+        HIP_TRY(hipMemsetAsync(f->d_steady, 0, px * 3 * sizeof(float), f->stream));
+    }
     HIP_TRY(hipStreamSynchronize(f->stream));
     std::fill(f->weight.begin(), f->weight.end(), 0.0);
     return TR_OK;
@@ -1384,7 +1476,17 @@ int tr_film_read_sums(tr_film f, double* sums, double* weights)
         HIP_TRY(hipStreamSynchronize(f->stream));
         for (size_t i = 0; i < f->n_elems; ++i) sums[i] = double(s[i]) + double(c[i]);
     }
-    if (weights) std::copy(f->weight.begin(), f->weight.end(), weights);
+    if (weights) {
+        if (f->layers) {
+            const size_t n = size_t(f->layers) * f->weight.size();
+            std::vector<float> lw(n);
+            HIP_TRY(hipMemcpyAsync(lw.data(), f->d_layer_w, n * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+            HIP_TRY(hipStreamSynchronize(f->stream));
+            for (size_t i = 0; i < n; ++i) weights[i] = double(lw[i]);
+        } else {
+            std::copy(f->weight.begin(), f->weight.end(), weights);
+        }
+    }
     return TR_OK;
 }
 
@@ -1396,8 +1498,32 @@ int tr_film_read(tr_film f, float* out, float* steady)
     HIP_TRY(hipMemcpyAsync(s.data(), f->d_sum, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
     HIP_TRY(hipMemcpyAsync(c.data(), f->d_comp, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
     HIP_TRY(hipStreamSynchronize(f->stream));
-    const size_t W = size_t(f->desc.width), T = size_t(f->desc.time_bins);
-    for (size_t p = 0; p < size_t(f->rows) * W; ++p) {
+    const size_t W = size_t(f->desc.width), T = size_t(f->desc.time_bins), P = size_t(f->rows) * W;
+    if (f->layers) {
+        // each bounce film divided by its own weights; the steady image holds every sample
+        std::vector<float> lw(size_t(f->layers) * P), st(P * 3);
+        HIP_TRY(hipMemcpyAsync(lw.data(), f->d_layer_w, lw.size() * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+        HIP_TRY(hipMemcpyAsync(st.data(), f->d_steady, st.size() * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+        HIP_TRY(hipStreamSynchronize(f->stream));
+        if (out)
+            for (size_t l = 0; l < size_t(f->layers); ++l)
+                for (size_t p = 0; p < P; ++p) {
+                    double w = lw[l * P + p];
+                    for (size_t t = 0; t < T * 3; ++t) {
+                        size_t i = ((l * P + p) * T) * 3 + t;
+                        double v = (double(s[i]) + double(c[i])) / w;
+                        out[i] = std::isfinite(v) ? float(v) : 0.0f;
+                    }
+                }
+        if (steady)
+            for (size_t p = 0; p < P; ++p)
+                for (int k = 0; k < 3; ++k) {
+                    double v = double(st[p * 3 + size_t(k)]) / f->weight[p];
+                    steady[p * 3 + size_t(k)] = std::isfinite(v) ? float(v) : 0.0f;
+                }
+        return TR_OK;
+    }
+    for (size_t p = 0; p < P; ++p) {
         double w = f->weight[p];
         double acc[3] = {0, 0, 0};
         for (size_t t = 0; t < T; ++t)
@@ -1427,10 +1553,12 @@ int tr_film_write(tr_film f, const char* name, const char* ext)
     const size_t slice = size_t(W) * size_t(T) * 3;
     std::vector<float> tr;
     char path[4096];
+    // multibounce films: bounce film b is "<name>_b%02d" (b + 1) (MultibounceStreakCameraFilm.h:127-135)
+    for (int l = 0; l < std::max(f->layers, 1); ++l)
     for (int r = 0; r < f->rows; ++r) {
         const int y = f->desc.scanline >= 0 ? f->desc.scanline : r;
         // a slice is an image T wide and W high; below 8 columns it is stored transposed
-        const float* data = img.data() + size_t(r) * slice;
+        const float* data = img.data() + (size_t(l) * f->rows + size_t(r)) * slice;
         int width = T, height = W;
         if (T < 8) {
             tr.resize(slice);
@@ -1440,7 +1568,8 @@ int tr_film_write(tr_film f, const char* name, const char* ext)
             data = tr.data();
             width = W, height = T;
         }
-        std::snprintf(path, sizeof path, "%s_%04d.%s", name, y, e.c_str());
+        if (f->layers) std::snprintf(path, sizeof path, "%s_b%02d_%04d.%s", name, l + 1, y, e.c_str());
+        else std::snprintf(path, sizeof path, "%s_%04d.%s", name, y, e.c_str());
         rc = e == "hdr" ? tri::write_rgbe(path, data, width, height) : tri::write_raw(path, data, slice);
         if (rc) return rc;
     }
@@ -1540,6 +1669,11 @@ int make_job(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* 
     R.ratio = float(fd.width) / float(fd.height);
     R.inv_spp = float(1. / double(float(d->sqrt_spp)));
     R.seed = d->seed;
+    R.layers = f->layers;
+    R.layer_stride = uint32_t(size_t(f->rows) * size_t(fd.width) * size_t(fd.time_bins));
+    R.steady = f->d_steady;
+    R.layer_w = f->d_layer_w;
+    R.cs_bits = nullptr;  // set by the launch (wavefront buffers)
     out->scene = s;
     out->film = f;
     out->dev.S = s->view();
@@ -1652,12 +1786,20 @@ struct Launch {
             if ((rc = begin(TR_K_FILM))) return rc;
             if (batch)
                 film_batch_kernel<<<std::max(1u, std::min((wave + kBlock - 1) / kBlock, cu * 16u)), kBlock, 0, stream>>>(dj, Wv);
+            else if (R0.layers > 0)
+                film_mb_kernel<<<std::max(1u, std::min((wave + kBlock - 1) / kBlock, cu * 16u)), kBlock, 0, stream>>>(R0, Wv);
             else if (guide)
                 film_kernel<true><<<fg, kFilmBlock, film_lds, stream>>>(R0, Wv, pix_slots, GV.contrib);
             else
                 film_kernel<false><<<fg, kFilmBlock, film_lds, stream>>>(R0, Wv, pix_slots, nullptr);
             if ((rc = end())) return rc;
             cur ^= 1;
+        }
+        if (!batch && R0.layers > 0) {
+            if ((rc = begin(TR_K_FILM))) return rc;
+            const uint32_t ncs = wave / uint32_t(R0.n_inc);
+            mb_weight_kernel<<<std::max(1u, std::min((ncs + kBlock - 1) / kBlock, cu * 16u)), kBlock, 0, stream>>>(R0, Wv);
+            if ((rc = end())) return rc;
         }
         return TR_OK;
     }
@@ -1680,10 +1822,25 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
         bounces = std::max(bounces, j.max_bounces);
         stack_bound = std::max(stack_bound, cfg.wide ? j.scene->stack4 : j.scene->depth);
     }
+    const bool mb = jobs[0].dev.R.layers > 0;
+    for (auto& j : jobs)
+        if (j.dev.R.layers > 0 && (batch || guide))
+            return fail(TR_ERR_INVALID, "render: multibounce films render alone and without guiding");
     LaunchCfg c = cfg;
     c.wave = uint32_t(std::min<uint64_t>(c.wave ? c.wave : kDefaultWave, std::max<uint64_t>(total, 1)));
+    if (mb) {
+        // wavefronts hold whole camera samples, whose layer marks are counted per wavefront
+        const uint32_t ni = uint32_t(jobs[0].dev.R.n_inc);
+        c.wave = std::max(ni, c.wave / ni * ni);
+    }
     int rc = wb.ensure(c.wave);
     if (rc) return rc;
+    if (mb) {
+        const uint32_t ncs = c.wave / uint32_t(jobs[0].dev.R.n_inc);
+        if ((rc = wb.ensure_tags()) || (rc = wb.ensure_cs(ncs))) return rc;
+        HIP_TRY(hipMemsetAsync(wb.cs_bits, 0, size_t(ncs) * sizeof(uint32_t), stream));
+        jobs[0].dev.R.cs_bits = wb.cs_bits;
+    }
     Launch L{};
     L.stream = stream;
     L.wb = &wb;
@@ -1768,6 +1925,7 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
             // count the kernels each replay runs
             L.launches[TR_K_CAMERA] += 1;
             for (int k : {TR_K_EXTEND, TR_K_SHADE, TR_K_SHADOW, TR_K_FILM}) L.launches[k] += uint32_t(bounces);
+            if (mb) L.launches[TR_K_FILM] += 1;
             hipError_t le = hipGraphLaunch(exec, stream);
             if (le != hipSuccess) {
                 drop_graph();

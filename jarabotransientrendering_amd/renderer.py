@@ -187,6 +187,8 @@ class StratifiedSampler:
 class StreakCameraFilm:
     """StreakCameraFilm (src/Film/StreakCameraFilm.h:40-437) with a box filter of size 1."""
 
+    bounce_layers = 0
+
     def __init__(self, width: int, height: int, time_bins: int, exposure: float, camera_unwarp: bool = False):
         self.width, self.height, self.time_bins = int(width), int(height), int(time_bins)
         self.exposure = float(exposure)
@@ -228,7 +230,7 @@ class StreakCameraFilm:
     def handle(self):
         if self._h is None:
             d = nat.FilmDesc(self.width, self.height, self.time_bins, self.exposure, self.offset, self.streak,
-                             int(self.camera_unwarp))
+                             int(self.camera_unwarp), int(self.bounce_layers))
             h = ctypes.c_void_p()
             nat.check(nat.lib().tr_film_create(ctypes.byref(h), ctypes.byref(d)))
             self._h = h
@@ -243,17 +245,22 @@ class StreakCameraFilm:
     def clean(self) -> None:
         nat.check(nat.lib().tr_film_clear(self.handle))
 
+    def _lead(self):
+        return (self.bounce_layers,) if self.bounce_layers else ()
+
     def read(self, steady: bool = False):
-        """Normalised streak data [rows][W][T][3] (and the steady image [rows][W][3])."""
-        out = np.empty((self.rows, self.width, self.time_bins, 3), dtype=np.float32)
+        """Normalised streak data [rows][W][T][3] (and the steady image [rows][W][3]); multibounce
+        films return [layers][rows][W][T][3]."""
+        out = np.empty(self._lead() + (self.rows, self.width, self.time_bins, 3), dtype=np.float32)
         st = np.empty((self.rows, self.width, 3), dtype=np.float32) if steady else None
         nat.check(nat.lib().tr_film_read(self.handle, nat.as_ptr(out), nat.as_ptr(st) if steady else None))
         return (out, st) if steady else out
 
     def read_sums(self):
-        """Compensated sums [rows][W][T][3] (float64) and weights [rows][W]."""
-        s = np.empty((self.rows, self.width, self.time_bins, 3), dtype=np.float64)
-        w = np.empty((self.rows, self.width), dtype=np.float64)
+        """Compensated sums [rows][W][T][3] (float64) and weights [rows][W] (multibounce films:
+        a leading layer axis on both)."""
+        s = np.empty(self._lead() + (self.rows, self.width, self.time_bins, 3), dtype=np.float64)
+        w = np.empty(self._lead() + (self.rows, self.width), dtype=np.float64)
         nat.check(nat.lib().tr_film_read_sums(self.handle, nat.as_ptr(s), nat.as_ptr(w)))
         return s, w
 
@@ -266,7 +273,23 @@ class StreakCameraFilm:
             os.makedirs(d, exist_ok=True)
         nat.check(nat.lib().tr_film_write(self.handle, name.encode(), extension.encode()))
         rows = [self.streak] if self.streak >= 0 else range(self.height)
+        if self.bounce_layers:
+            return [f"{name}_b{b + 1:02d}_{y:04d}.{extension}" for b in range(self.bounce_layers) for y in rows] + \
+                [f"{name}.hdr"]
         return [f"{name}_{y:04d}.{extension}" for y in rows] + [f"{name}.hdr"]
+
+
+class MultibounceStreakFilm(StreakCameraFilm):
+    """MultibounceStreakFilm (src/Film/MultibounceStreakCameraFilm.h:36-200): one streak film per
+    bounce (samples of bounce >= nb_bounces go to the last), each normalised by the camera samples
+    that sent it samples, and a steady image of every sample."""
+
+    def __init__(self, width: int, height: int, time_bins: int, exposure: float, nb_bounces: int,
+                 camera_unwarp: bool = False):
+        super().__init__(width, height, time_bins, exposure, camera_unwarp)
+        if not 1 <= int(nb_bounces) <= 32:
+            raise ValueError("nb_bounces must be in [1, 32]")
+        self.bounce_layers = int(nb_bounces)
 
 
 class BidirectionalPathTracing:
@@ -478,7 +501,10 @@ def setup_scene(scene):
     cam = PinholeCamera(c.position, c.focus, c.up, c.fov_deg)
     f = scene.film
     sampler = StratifiedSampler(f.width, f.height, scene.render.sqrt_spp)
-    film = StreakCameraFilm(f.width, f.height, f.time_bins, f.exposure, f.camera_unwarp)
+    if getattr(f, "bounce_layers", 0):
+        film = MultibounceStreakFilm(f.width, f.height, f.time_bins, f.exposure, f.bounce_layers, f.camera_unwarp)
+    else:
+        film = StreakCameraFilm(f.width, f.height, f.time_bins, f.exposure, f.camera_unwarp)
     if f.offset > 0:
         film.set_offset(f.offset)
     if f.scanline >= 0:

@@ -40,6 +40,7 @@ class Film:
     offset: float = 0.0
     scanline: int = -1          # -film-scanline y (bunnykiller.cpp:1082-1086); -1 = whole film
     camera_unwarp: bool = False  # -camera-unwarp (bunnykiller.cpp:1442)
+    bounce_layers: int = 0       # -multibounce-streak N (bunnykiller.cpp:1413-1417); 0 = one streak film
 
 
 @dataclasses.dataclass
@@ -99,6 +100,8 @@ class Scene:
             lines.append("rect " + " ".join(repr(float(v)) for v in q))
         if self.medium is not None:
             lines.append("medium " + " ".join(repr(float(v)) for v in self.medium))
+        if f.bounce_layers:
+            lines.append(f"multibounce {int(f.bounce_layers)}")
         lines.append(f"out {out_prefix}")
         return "\n".join(lines) + "\n"
 

@@ -726,6 +726,15 @@ static int connect_paths(ctx_t* c, rngs* R, const vertex_t* verts, int nv, int m
 int orc_render(const orc_scene* s, const orc_render_desc* d, double* film_sum, double* weight,
                orc_stats* stats)
 {
+    if (d && d->bounce_layers > 0) return -1;
+    return orc_render_ex(s, d, film_sum, weight, NULL, NULL, stats);
+}
+
+int orc_render_ex(const orc_scene* s, const orc_render_desc* d, double* film_sum, double* weight,
+                  double* layer_weight, double* steady, orc_stats* stats)
+{
+    const int B = d ? d->bounce_layers : 0;
+    if (B < 0 || B > 32 || (B > 0 && !layer_weight)) return -1;
     if (!s || !d || d->width <= 0 || d->height <= 0 || d->time_bins <= 0 || d->sqrt_spp <= 0 ||
         d->n_incoming <= 0 || (s->n_points + s->n_rects) <= 0)
         return -1;
@@ -830,18 +839,31 @@ int orc_render(const orc_scene* s, const orc_render_desc* d, double* film_sum, d
                         if (nvx == 0) continue;
                         ns += connect_paths(&c, &R, verts, nvx, max_size, inv_inc, smp + ns, &st);
                     }
+                    uint32_t layers_seen = 0;
+                    const size_t pix = (size_t)(y - row0) * (size_t)W + (size_t)x;
+                    const size_t layer_stride = (size_t)(row1 - row0) * (size_t)W * (size_t)T * 3;
                     for (int k = 0; k < ns; ++k) {
                         float* rgb = smp[k].rgb;
                         if (!isfinite(rgb[0]) || !isfinite(rgb[1]) || !isfinite(rgb[2]))
                             rgb[0] = rgb[1] = rgb[2] = 0.0f;   /* TransientRenderer.h:167-171 */
+                        int layer = 0;
+                        if (B > 0) {
+                            layer = smp[k].bounce < B ? smp[k].bounce : B - 1;
+                            layers_seen |= 1u << layer;
+                        }
+                        if (steady) {
+                            steady[pix * 3 + 0] += rgb[0]; steady[pix * 3 + 1] += rgb[1]; steady[pix * 3 + 2] += rgb[2];
+                        }
                         float cam_time = d->camera_unwarp ? smp[k].time - rec_dist : smp[k].time;
                         float pos = (cam_time - d->offset) / d->exposure;
                         if (!(pos < (float)T && pos >= 0.0f)) continue;
                         size_t bin = (size_t)floorf(pos);
                         if (bin >= (size_t)T) continue;
-                        double* dst = film_sum + (((size_t)(y - row0) * (size_t)W + (size_t)x) * (size_t)T + bin) * 3;
+                        double* dst = film_sum + (size_t)layer * layer_stride + (pix * (size_t)T + bin) * 3;
                         dst[0] += rgb[0]; dst[1] += rgb[1]; dst[2] += rgb[2];
                     }
+                    for (int b = 0; b < B; ++b)
+                        if (layers_seen & (1u << b)) layer_weight[(size_t)b * (size_t)(row1 - row0) * (size_t)W + pix] += 1.0;
                 }
             }
         }

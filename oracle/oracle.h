@@ -60,6 +60,9 @@ typedef struct {
      * (Ray.h:36) for closest and shadow hits and shortens shadow rays by 1e-5 * scene radius
      * (World.h:687-691). */
     float tnear, shadow_tnear, shadow_tfar_eps;
+    /* MultibounceStreakFilm (MultibounceStreakCameraFilm.h:67-171) with this many bounce films;
+     * 0 = a single StreakCameraFilm */
+    int32_t bounce_layers;
 } orc_render_desc;
 
 typedef struct {
@@ -84,6 +87,15 @@ int orc_intersect(const orc_scene* s, const float* rays, int32_t n, float tnear,
  * Rows: scanline mode -> 1 row; else [row_begin, row_end). */
 int orc_render(const orc_scene* s, const orc_render_desc* d, double* film_sum, double* weight,
                orc_stats* stats);
+
+/* orc_render with the multibounce outputs. With d->bounce_layers = B > 0, film_sum is
+ * [B][rows][W][T][3]: a sample of bounce b goes to layer min(b, B - 1); layer_weight[B][rows][W]
+ * counts, per pixel, the camera samples that produced at least one sample of that layer (a bounce
+ * film only sees camera samples with samples of its bounce, MultibounceStreakCameraFilm.h:137-157);
+ * steady[rows][W][3] (optional) sums every sample regardless of the time window (Film::add_samples,
+ * Film.h:263-283). weight[rows][W] counts all camera samples. */
+int orc_render_ex(const orc_scene* s, const orc_render_desc* d, double* film_sum, double* weight,
+                  double* layer_weight, double* steady, orc_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,7 @@ class _Render(ctypes.Structure):
         ("seed", ctypes.c_uint64), ("rng_mode", ctypes.c_int32),
         ("row_begin", ctypes.c_int32), ("row_end", ctypes.c_int32),
         ("tnear", ctypes.c_float), ("shadow_tnear", ctypes.c_float), ("shadow_tfar_eps", ctypes.c_float),
+        ("bounce_layers", ctypes.c_int32),
     ]
 
 
@@ -75,6 +76,9 @@ def lib():
         L.orc_render.argtypes = [ctypes.POINTER(_Scene), ctypes.POINTER(_Render), ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.POINTER(Stats)]
         L.orc_render.restype = ctypes.c_int
+        L.orc_render_ex.argtypes = [ctypes.POINTER(_Scene), ctypes.POINTER(_Render), ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Stats)]
+        L.orc_render_ex.restype = ctypes.c_int
         L.orc_intersect.argtypes = [ctypes.POINTER(_Scene), ctypes.c_void_p, ctypes.c_int32, ctypes.c_float,
                                     ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_intersect.restype = ctypes.c_int
@@ -135,8 +139,11 @@ def _scene_struct(scene):
 NON_EMBREE_EPS = (1e-4, 1e-4, 0.5e-5)
 
 
-def render(scene, rng_mode=RNG_PER_PATH, rows=None, seed=None, eps=None):
-    """Returns (film_sum [rows][W][T][3] float64, weight [rows][W] float64, stats dict)."""
+def render(scene, rng_mode=RNG_PER_PATH, rows=None, seed=None, eps=None, bounce_layers=0, extra=None):
+    """Returns (film_sum [rows][W][T][3] float64, weight [rows][W] float64, stats dict).
+    With bounce_layers = B > 0 the film is [B][rows][W][T][3] (MultibounceStreakFilm layers) and,
+    if `extra` is a dict, extra["layer_weight"] ([B][rows][W]) and extra["steady"] ([rows][W][3],
+    every sample regardless of the time window) are filled in."""
     s, keep = _scene_struct(scene)
     f, c, r = scene.film, scene.camera, scene.render
     d = _Render()
@@ -156,11 +163,18 @@ def render(scene, rng_mode=RNG_PER_PATH, rows=None, seed=None, eps=None):
         r0, r1 = rows if rows is not None else (0, f.height)
         d.row_begin, d.row_end = r0, r1
         nrows = r1 - r0
-    film = np.zeros((nrows, f.width, f.time_bins, 3), dtype=np.float64)
+    d.bounce_layers = int(bounce_layers)
+    shape = (nrows, f.width, f.time_bins, 3)
+    film = np.zeros(((bounce_layers,) if bounce_layers else ()) + shape, dtype=np.float64)
     weight = np.zeros((nrows, f.width), dtype=np.float64)
+    lw = np.zeros((max(bounce_layers, 1), nrows, f.width), dtype=np.float64)
+    steady = np.zeros((nrows, f.width, 3), dtype=np.float64)
     st = Stats()
-    rc = lib().orc_render(ctypes.byref(s), ctypes.byref(d), film.ctypes.data, weight.ctypes.data,
-                          ctypes.byref(st))
+    rc = lib().orc_render_ex(ctypes.byref(s), ctypes.byref(d), film.ctypes.data, weight.ctypes.data,
+                             lw.ctypes.data, steady.ctypes.data, ctypes.byref(st))
+    if extra is not None:
+        extra["layer_weight"] = lw[:bounce_layers] if bounce_layers else None
+        extra["steady"] = steady
     if rc != 0:
         raise ValueError(f"orc_render failed with status {rc}")
     del keep

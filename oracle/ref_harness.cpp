@@ -53,6 +53,7 @@
 #include "RayTracing/World.h"
 #include "Renderer/TransientRenderer.h"
 #include "Film/StreakCameraFilm.h"
+#include "Film/MultibounceStreakCameraFilm.h"
 #include "Filter/BoxFilter.h"
 #include "Geometry/Aggregate/Mesh.h"
 #include "Geometry/ObjectPointer.h"
@@ -83,6 +84,7 @@ struct SceneDesc {
     std::vector<std::array<double, 9>> rects;
     bool has_medium = false;
     double medium[6] = {0, 0, 0, 0, 0, 0};
+    int multibounce = 0;  // > 0: MultibounceStreakFilm with this many bounce films (scanline only)
     std::string out = "film";
 };
 
@@ -113,6 +115,8 @@ static SceneDesc parse_scene(const char* path)
             std::array<double, 9> r; for (double& v : r) ls >> v; s.rects.push_back(r);
         } else if (key == "medium") {
             s.has_medium = true; for (double& v : s.medium) ls >> v;
+        } else if (key == "multibounce") {
+            ls >> s.multibounce;
         } else if (key == "out") {
             ls >> s.out;
         } else {
@@ -132,6 +136,62 @@ static std::vector<Triangle> make_triangles(const SceneDesc& s, std::vector<Mate
         tris.push_back(Triangle(v0, v2, v1, mats[s.tri_mat[i]]));
     }
     return tris;
+}
+
+// Read access to the bounce films of a MultibounceStreakFilm and to protected members of a
+// StreakCameraFilm, to dump the in-memory streak slice and weights of each bounce film.
+using StreakR = StreakCameraFilm<3, Radiance>;
+struct MBProbe : MultibounceStreakFilm<3, Radiance> {
+    using MultibounceStreakFilm<3, Radiance>::MultibounceStreakFilm;
+    StreakR* bounce(size_t b) { return m_bounce_films[b].get(); }
+};
+struct StreakProbe : StreakR {
+    static Imaging::Image<Real> Film<3, Radiance>::*weight() { return &StreakProbe::m_weight; }
+    static size_t StreakR::*first_slice() { return &StreakProbe::m_first_available_slice; }
+};
+
+static int run_render_mb(const SceneDesc& s, World<3, Radiance>& w, PinholeCamera& camera)
+{
+    if (s.scanline < 0) { fprintf(stderr, "multibounce dumps need a scanline film\n"); return 2; }
+    StratifiedSampler sampler(s.W, s.H, s.sqrt_spp, true);
+    MBProbe film(s.W, s.H, s.T, s.exposure, (unsigned)s.multibounce, new BoxFilter(1.), s.unwarp != 0,
+                 FilmComponents::RADIANCE);
+    film.set_name(s.out.c_str(), "raw");
+    if (s.offset > 0.0) film.set_offset(s.offset);
+    film.set_aspect_ratio(1.);
+    sampler.set_scanline(s.scanline);
+    film.set_streak(s.scanline);
+    Random::StdRNG.seed(s.seed);
+    FILE* f_log = fopen((s.out + ".log").c_str(), "w+");
+    BidirectionalPathTracing<3, Radiance, RadianceAttenuation> bpt(w, s.n_incoming, f_log, &film);
+    bpt.set_max_path_size(s.max_bounces);
+    bpt.set_path_tracing_only(true);
+    bpt.set_scattering_components(Scattering::ALL);
+    bpt.set_mode(BidirectionalPathTracing<3, Radiance, RadianceAttenuation>::SamplingMode::SteadyState);
+    TransientRenderer<3, Radiance, RadianceAttenuation> engine(f_log, false);
+    engine.render(s.out.c_str(), w, &bpt, camera, &film, &sampler);
+    if (f_log) fclose(f_log);
+    // per bounce film: the unnormalised slice [W][T][3] and the weights [W] of the scanline
+    FILE* fs = fopen((s.out + "_mb.f32").c_str(), "wb");
+    FILE* fw = fopen((s.out + "_mbw.f32").c_str(), "wb");
+    for (int b = 0; b < s.multibounce; ++b) {
+        StreakR* f = film.bounce((size_t)b);
+        const bool has_slice = (f->*StreakProbe::first_slice()) == (size_t)s.scanline;
+        for (int x = 0; x < s.W; ++x)
+            for (int t = 0; t < s.T; ++t)
+                for (int c = 0; c < 3; ++c) {
+                    float v = has_slice ? (float)f->streak_image(s.scanline)(t, x, c) : 0.f;
+                    fwrite(&v, 4, 1, fs);
+                }
+        const Imaging::Image<Real>& wimg = f->*StreakProbe::weight();
+        for (int x = 0; x < s.W; ++x) {
+            float v = (float)wimg(x, s.scanline, 0);
+            fwrite(&v, 4, 1, fw);
+        }
+    }
+    fclose(fs);
+    fclose(fw);
+    return 0;
 }
 
 static int run_render(const SceneDesc& s, bool reseed_per_camera_sample)
@@ -174,6 +234,7 @@ static int run_render(const SceneDesc& s, bool reseed_per_camera_sample)
 
     w.set_background(Spectrum(0.));
     w.freeze();
+    if (s.multibounce > 0) return run_render_mb(s, w, camera);
 
     StratifiedSampler sampler(s.W, s.H, s.sqrt_spp, true);
     StreakCameraFilm<3, Radiance> film(s.W, s.H, s.T, s.exposure, s.unwarp != 0, new BoxFilter(1.),

@@ -14,6 +14,8 @@ scenes below and records:
   stats_<name>.npz               per-bin mean / standard deviation over K seeds
   isect_<name>.npz               closest-hit triangle ids and distances of seeded
                                  rays from the reference's Triangle::intersect
+  mb_<name>.npz                  MultibounceStreakFilm bounce films of a scanline: the
+                                 un-normalised slices [B][W][T][3] and weights [B][W]
   image_<name>.npz               an RGB float image and the Radiance .hdr bytes the
                                  reference's Imaging::save wrote for it
   manifest.json                  scene parameters, harness mode, seeds, and a
@@ -109,6 +111,35 @@ def isect_cases():
     return out
 
 
+def mb_cases():
+    box = scenes.cornell_box(width=16, height=16, time_bins=128, exposure=0.02, sqrt_spp=3, n_incoming=4,
+                             max_bounces=5, seed=3)
+    box.film.scanline, box.film.bounce_layers = 9, 3
+    rect = scenes.cornell_box(width=12, height=12, time_bins=64, exposure=0.05, sqrt_spp=2, n_incoming=6,
+                              max_bounces=6, seed=5)
+    rect.point_lights = np.zeros((0, 6), dtype=np.float32)
+    rect.rect_lights = np.array([[0.65, 0.99, 0.65, 0.65, 0.0, 0.65, 0.3, 0.3, 20.0]], dtype=np.float32)
+    rect.film.scanline, rect.film.bounce_layers, rect.film.camera_unwarp = 4, 4, True
+    rect.name = "box_rect_unwarp"
+    return [("box", box), ("rect_unwarp", rect)]
+
+
+def write_mb(manifest=None):
+    with tempfile.TemporaryDirectory() as tmp:
+        for tag, s in mb_cases():
+            prefix = os.path.join(tmp, "mb_" + tag)
+            with open(prefix + ".txt", "w") as fh:
+                fh.write(s.to_ref_text(prefix))
+            subprocess.check_call([HARNESS, "render", prefix + ".txt"], stdout=subprocess.DEVNULL)
+            B, W, T = s.film.bounce_layers, s.film.width, s.film.time_bins
+            sums = np.fromfile(prefix + "_mb.f32", np.float32).reshape(B, W, T, 3)
+            w = np.fromfile(prefix + "_mbw.f32", np.float32).reshape(B, W)
+            np.savez_compressed(os.path.join(HERE, f"mb_{tag}.npz"), sums=sums, weights=w)
+            if manifest is not None:
+                manifest.setdefault("multibounce", {})[tag] = {"mode": "render", **describe(s)}
+            print("mb", tag, sums.shape, float(sums.sum()), w.sum(axis=1))
+
+
 def image_cases():
     """Float RGB images for the RGBE writer: narrow rows (stored without run-length encoding),
     long rows with runs, literal spans, zeros, a huge dynamic range and negative values."""
@@ -129,6 +160,13 @@ def main(only=None):
     subprocess.check_call(["sh", os.path.join(ROOT, "oracle", "build_ref.sh")])
     if only == "images":
         write_images()
+        return
+    if only == "multibounce":
+        with open(os.path.join(HERE, "manifest.json")) as fh:
+            manifest = json.load(fh)
+        write_mb(manifest)
+        with open(os.path.join(HERE, "manifest.json"), "w") as fh:
+            json.dump(manifest, fh, indent=1, default=float)
         return
     manifest = {"generator": "tests/golden/make_golden.py", "harness": "oracle/ref_harness.cpp",
                 "host": platform.processor() or platform.machine(), "films": {}, "stats": {}, "isect": {}}
@@ -168,6 +206,7 @@ def main(only=None):
             manifest["isect"][tag] = {"n_rays": int(rays.shape[0]), **describe(s)}
             print(tag, "isect", int((hits[:, 0] >= 0).sum()), "hits")
     write_images()
+    write_mb(manifest)
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, default=float)
 

@@ -42,7 +42,7 @@ def test_struct_layouts_match_the_header():
     from jarabotransientrendering_amd import _native as nat
     assert ctypes.sizeof(nat.Ray) == 32
     assert ctypes.sizeof(nat.Hit) == 16
-    assert ctypes.sizeof(nat.FilmDesc) == 28
+    assert ctypes.sizeof(nat.FilmDesc) == 32
     assert ctypes.sizeof(nat.CameraDesc) == 44
     assert ctypes.sizeof(nat.RenderDesc) == 32
     assert ctypes.sizeof(nat.RenderStats) == 8 * 4 + 4 * 2 + 8 + 8 * 8 + 4 * 8
