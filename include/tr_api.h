@@ -63,6 +63,29 @@ int tr_scene_destroy(tr_scene scene);
 int tr_scene_add_mesh(tr_scene scene, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
                       const tr_material* material, uint32_t* geom_id);
 
+enum {
+    /* -two-sided: TwoSidedMaterial around the Lambertian (src/Material/TwoSided.h:34-96): hit from
+     * behind, the shading frame is inverted, and cosines are taken as |cos| (Path.h:188-193) */
+    TR_MATERIAL_TWO_SIDED = 1u
+};
+
+/* tr_scene_add_mesh with per-corner shading normals (`normals`: 9 floats per triangle, corners in
+ * index order; NULL = the geometric normal) interpolated at each hit as the reference does with
+ * OBJ "vn" data (World.h:490-509, 589-598; not renormalised), and material flags. */
+int tr_scene_add_mesh_ex(tr_scene scene, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                         const float* normals, const tr_material* material, uint32_t flags, uint32_t* geom_id);
+
+/* -name-mesh file.obj: World::add_triangle_mesh (src/RayTracing/World.h:209-541) reading a
+ * Wavefront OBJ (triangles only; "f v", "v/vt", "v//vn", "v/vt/vn" corners; "vn" normals are
+ * normalised and used as shading normals). Errors carry the reference's messages. */
+int tr_scene_add_obj(tr_scene scene, const char* path, const tr_material* material, uint32_t flags,
+                     uint32_t* geom_id);
+
+/* The OBJ reader alone (host only): vertices, triangle indices and, if the file has "vn" data,
+ * per-corner normals (9 floats per triangle; *normals = NULL otherwise). Release with tr_obj_free. */
+int tr_obj_load(const char* path, float** xyz, uint32_t* nv, uint32_t** idx, uint32_t* nt, float** normals);
+int tr_obj_free(float* xyz, uint32_t* idx, float* normals);
+
 /* PointLightSource (src/LightSource/PointLightSource.h:39-171; CLI bunnykiller.cpp:1194-1207). */
 int tr_scene_add_point_light(tr_scene scene, const float position[3], const float intensity[3]);
 

@@ -18,6 +18,7 @@ TR_ERR_DEVICE = -2
 TR_ERR_NOMEM = -3
 TR_ERR_STATE = -4
 
+TR_MATERIAL_TWO_SIDED = 1
 TR_RENDER_PROFILE = 1
 TR_RENDER_BINARY_BVH = 2
 TR_RENDER_ORTHONORMAL_FRAME = 4
@@ -115,6 +116,15 @@ SIGNATURES = {
     "tr_scene_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "tr_scene_add_mesh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                          ctypes.c_uint32, ctypes.POINTER(Material), ctypes.POINTER(ctypes.c_uint32)]),
+    "tr_scene_add_mesh_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                            ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(Material),
+                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+    "tr_scene_add_obj": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(Material), ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_uint32)]),
+    "tr_obj_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(ctypes.c_float)),
+                                   ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32)),
+                                   ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.POINTER(ctypes.c_float))]),
+    "tr_obj_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "tr_scene_add_point_light": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float * 3, ctypes.c_float * 3]),
     "tr_scene_add_rect_light": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float * 3, ctypes.c_float * 3,
                                                ctypes.c_float * 2, ctypes.c_float]),
@@ -199,3 +209,23 @@ def device_info():
 
 def as_ptr(a: np.ndarray):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def obj_load(path: str):
+    """tr_obj_load: (vertices (nv,3), triangles (nt,3), corner normals (nt,3,3) or None)."""
+    xyz = ctypes.POINTER(ctypes.c_float)()
+    idx = ctypes.POINTER(ctypes.c_uint32)()
+    nrm = ctypes.POINTER(ctypes.c_float)()
+    nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+    check(lib().tr_obj_load(path.encode(), ctypes.byref(xyz), ctypes.byref(nv), ctypes.byref(idx), ctypes.byref(nt),
+                            ctypes.byref(nrm)))
+    try:
+        v = np.ctypeslib.as_array(xyz, shape=(max(nv.value, 1) * 3,))[:nv.value * 3].reshape(-1, 3).copy()
+        t = np.ctypeslib.as_array(idx, shape=(max(nt.value, 1) * 3,))[:nt.value * 3].reshape(-1, 3).copy()
+        n = None
+        if nrm:
+            n = np.ctypeslib.as_array(nrm, shape=(nt.value * 9,)).reshape(-1, 3, 3).copy()
+    finally:
+        lib().tr_obj_free(ctypes.cast(xyz, ctypes.c_void_p), ctypes.cast(idx, ctypes.c_void_p),
+                          ctypes.cast(nrm, ctypes.c_void_p))
+    return v, t, n

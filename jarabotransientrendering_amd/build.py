@@ -11,7 +11,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(LIBDIR, "libtransient_hip.so")
 
-SOURCES = ["tr_kernels.hip", "tr_guide.hip", "tr_bvh.cpp", "tr_image_io.cpp"]
+SOURCES = ["tr_kernels.hip", "tr_guide.hip", "tr_bvh.cpp", "tr_image_io.cpp", "tr_obj.cpp"]
 HEADERS = ["tr_device.h", "tr_internal.h", "tr_guide.h", "tr_guide_host.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

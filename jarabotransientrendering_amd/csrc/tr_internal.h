@@ -84,3 +84,14 @@ void rgbe_from_float(float r, float g, float b, uint8_t out[4]);
 int write_rgbe(const char* path, const float* rgb, int width, int height);
 int write_raw(const char* path, const float* data, size_t count);
 }  // namespace tri
+
+// ---- OBJ meshes (tr_obj.cpp) -----------------------------------------------------------------
+namespace tri {
+struct ObjMesh {
+    std::vector<float> xyz;      // vertices, 3 floats each
+    std::vector<uint32_t> idx;   // triangles, 3 vertex indices each
+    std::vector<float> normals;  // per triangle corner shading normal, 9 floats per triangle
+    bool has_normals = false;    // some corner had a "vn"
+};
+int load_obj(const char* path, ObjMesh* out);
+}  // namespace tri

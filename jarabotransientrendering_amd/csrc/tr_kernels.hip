@@ -60,6 +60,10 @@ constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInvPi = 0.318309886183790671538f;
 constexpr float kInv4Pi = 0.0795774715459476678844f;  // .25 * M_1_PI (IsotropicPhaseFunction.h:43-47)
 constexpr float kSigmaVisibility = 1e-5f;             // _SIGMA_VISIBILITY_ (bunnykiller.h:76-78)
+// material bits of a scene triangle: index | two-sided material | smooth (per-corner normals)
+constexpr int32_t kMatTwoSided = 1 << 30;
+constexpr int32_t kMatSmooth = 1 << 29;
+constexpr int32_t kMatIndex = kMatSmooth - 1;
 
 // counters[] layout (uint32 words)
 enum {
@@ -73,7 +77,8 @@ struct SceneDev {
     const float4* nodes;       // binary BVH (4 float4 per node)
     const float4* nodes4;      // 4-wide BVH (8 float4 per node)
     const float4* tris;
-    const float4* tri_normal;  // per scene triangle: shading normal xyz, w = material index (int bits)
+    const float4* tri_normal;  // per scene triangle: geometric normal xyz, w = material bits (int)
+    const float4* tri_vn;      // smooth triangles: p0, e1, e2, n0, n1, n2 (null if none)
     const float4* mats;        // Kd xyz, w = mean albedo used by the roulette
     const LightHost* lights;
     int32_t n_lights;
@@ -293,6 +298,18 @@ __device__ __forceinline__ f3 sample_lambert_onb(f3 n, float e1, float e2, float
     return normalize(t * (cp * st) + bt * (sp * st) + n * ct);
 }
 
+// Interpolated shading normal of a smooth triangle at point p (per-corner normals n0, n1, n2 with
+// the barycentrics of p), not renormalised, as the reference uses it (World.h:589-598,
+// Triangle.h:155-159).
+__device__ __forceinline__ f3 smooth_normal(const float4* V, f3 p)
+{
+    f3 p0 = mk(V[0]), e1 = mk(V[1]), e2 = mk(V[2]), q = p - p0;
+    float d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2), d20 = dot(q, e1), d21 = dot(q, e2);
+    float inv = 1.0f / (d00 * d11 - d01 * d01);
+    float u = (d11 * d20 - d01 * d21) * inv, v = (d00 * d21 - d01 * d20) * inv;
+    return mk(V[3]) * (1.0f - u - v) + mk(V[4]) * u + mk(V[5]) * v;
+}
+
 // Isotropic phase-function direction: uniform sphere rotated so that +y maps onto wi
 // (Sampling.h:80-94, IsotropicPhaseFunction.h:54-60, VectorN.h:429-523).
 __device__ __forceinline__ f3 sample_isotropic(f3 wi, float e1, float e2)
@@ -382,7 +399,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
         uint32_t sh_tag = 0xFFFFFFFFu, tag = 0xFFFFFFFFu;
         // state of a vertex that goes on to sample a direction
         f3 f = {}, pos = {}, n = {}, d = {};
-        bool medium_event = false;
+        bool medium_event = false, two_sided = false;
         int32_t mat = -1;
         float pi = 1.0f, time = 0.0f, nsampled = 0.0f;
         uint64_t pst = 0, nst = 0, pinc = 0;
@@ -428,6 +445,9 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                     float4 nn = S.tri_normal[tri];
                     n = mk(nn);
                     mat = __float_as_int(nn.w);
+                    two_sided = (mat & kMatTwoSided) != 0;
+                    if (mat & kMatSmooth) n = smooth_normal(S.tri_vn + 6 * size_t(tri), pos);
+                    mat &= kMatIndex;
                 }
                 nst = u64(extra.x, extra.y);
                 uint64_t pid = u64(misc.x, misc.y);
@@ -472,7 +492,9 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                             fe = mk(S.sigma_s.x, S.sigma_s.y, S.sigma_s.z) * kInv4Pi;
                         } else {
                             float4 kd = S.mats[mat];
-                            float c = fmaxf(-dot(ldir, n), 0.0f);
+                            // dot_clamped, or dot_abs for a two-sided material (Path.h:188-193)
+                            float cd = -dot(ldir, n);
+                            float c = two_sided ? fabsf(cd) : fmaxf(cd, 0.0f);
                             fe = mk(kd.x, kd.y, kd.z) * (kInvPi * c);
                         }
                         f3 att = mk(1.0f, 1.0f, 1.0f);
@@ -574,18 +596,22 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                     float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
                     float ct;
                     nd = MODE == kShadeRef ? sample_lambert(n, e1, e2, &ct) : sample_lambert_onb(n, e1, e2, &ct);
+                    // a two-sided material hit from behind samples in its inverted frame
+                    // (TwoSided.h:80-88, Intersection.h:180-185)
+                    if (two_sided && dot(d, n) > 0.0f) nd = nd * -1.0f;
                     float4 kd = S.mats[mat];
                     fa = mk(kd.x, kd.y, kd.z) * (kInvPi * fabsf(dot(n, nd)));
                     pa = ct * kInvPi;
                 }
             } else {
                 const float alpha = G.alpha;
+                const f3 nside = (two_sided && dot(d, n) > 0.0f) ? n * -1.0f : n;  // side the path arrives on
                 float usel = pcg_float(pst, pinc);
                 float e1, e2;
                 if (usel < alpha) {
                     e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
                     float ct;
-                    nd = medium_event ? sample_isotropic(d, e1, e2) : sample_lambert_onb(n, e1, e2, &ct);
+                    nd = medium_event ? sample_isotropic(d, e1, e2) : sample_lambert_onb(nside, e1, e2, &ct);
                 }
                 const trg::MixtureRef mix = trg::mixture_ref(mine + __lane_id() * trg::kRow);
                 if (!(usel < alpha)) {
@@ -599,7 +625,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                     fa = mk(S.sigma_s.x, S.sigma_s.y, S.sigma_s.z) * kInv4Pi;
                     pa = alpha * kInv4Pi + (1.0f - alpha) * q;
                 } else {
-                    float c = dot(n, nd);
+                    float c = dot(nside, nd);
                     ok = c > 0.0f;  // below the surface the one-sided Lambertian reflects nothing
                     float4 kd = S.mats[mat];
                     fa = mk(kd.x, kd.y, kd.z) * (kInvPi * fmaxf(c, 0.0f));
@@ -1054,8 +1080,9 @@ int compute_units()
 
 struct tr_scene_s {
     std::vector<float> tris;        // n*9
-    std::vector<int32_t> tri_mat;
+    std::vector<int32_t> tri_mat;   // material index | kMatTwoSided | kMatSmooth
     std::vector<uint32_t> tri_geom, tri_prim;
+    std::vector<float> tri_vn;      // per triangle corner shading normals (9 floats), once a mesh has some
     std::vector<float> mats;        // n_mat*4
     std::vector<LightHost> lights;
     uint32_t n_geoms = 0;
@@ -1064,6 +1091,7 @@ struct tr_scene_s {
     bool committed = false;
     hipStream_t stream = nullptr;
     float4 *d_nodes = nullptr, *d_nodes4 = nullptr, *d_tris = nullptr, *d_tri_normal = nullptr, *d_mats = nullptr;
+    float4* d_tri_vn = nullptr;  // smooth triangles: p0, e1, e2, n0, n1, n2 (6 float4 per triangle)
     LightHost* d_lights = nullptr;
     uint2* d_ids = nullptr;
     uint32_t n_nodes = 0, n_nodes4 = 0;
@@ -1081,6 +1109,7 @@ struct tr_scene_s {
     void release_device()
     {
         dev_free(d_nodes); dev_free(d_nodes4); dev_free(d_tris); dev_free(d_tri_normal); dev_free(d_mats); dev_free(d_lights);
+        dev_free(d_tri_vn);
         dev_free(d_ids);
         dev_bytes = 0;
     }
@@ -1091,6 +1120,7 @@ struct tr_scene_s {
         s.nodes4 = d_nodes4;
         s.tris = d_tris;
         s.tri_normal = d_tri_normal;
+        s.tri_vn = d_tri_vn;
         s.mats = d_mats;
         s.lights = d_lights;
         s.n_lights = int32_t(lights.size());
@@ -1170,7 +1200,25 @@ int tr_scene_destroy(tr_scene s)
 int tr_scene_add_mesh(tr_scene s, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
                       const tr_material* m, uint32_t* geom_id)
 {
+    return tr_scene_add_mesh_ex(s, xyz, nv, idx, nt, nullptr, m, 0u, geom_id);
+}
+
+int tr_scene_add_obj(tr_scene s, const char* path, const tr_material* m, uint32_t flags, uint32_t* geom_id)
+{
+    if (!s || !path || !m) return fail(TR_ERR_INVALID, "tr_scene_add_obj: null argument");
+    tri::ObjMesh mesh;
+    int rc = tri::load_obj(path, &mesh);
+    if (rc) return rc;
+    return tr_scene_add_mesh_ex(s, mesh.xyz.data(), uint32_t(mesh.xyz.size() / 3), mesh.idx.data(),
+                                uint32_t(mesh.idx.size() / 3), mesh.has_normals ? mesh.normals.data() : nullptr, m,
+                                flags, geom_id);
+}
+
+int tr_scene_add_mesh_ex(tr_scene s, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                         const float* normals, const tr_material* m, uint32_t flags, uint32_t* geom_id)
+{
     if (!s || (nt > 0 && (!xyz || !idx)) || !m) return fail(TR_ERR_INVALID, "tr_scene_add_mesh: null argument");
+    if (flags & ~uint32_t(TR_MATERIAL_TWO_SIDED)) return fail(TR_ERR_INVALID, "tr_scene_add_mesh: unknown flags");
     for (uint32_t i = 0; i < 3 * nt; ++i)
         if (idx[i] >= nv) return fail(TR_ERR_INVALID, "tr_scene_add_mesh: vertex index out of range");
     for (int k = 0; k < 3; ++k)
@@ -1181,14 +1229,24 @@ int tr_scene_add_mesh(tr_scene s, const float* xyz, uint32_t nv, const uint32_t*
     float alb = ((1.f - (1.f - m->kd[0])) + (1.f - (1.f - m->kd[1])) + (1.f - (1.f - m->kd[2]))) / 3.0f;
     s->mats.insert(s->mats.end(), {m->kd[0], m->kd[1], m->kd[2], alb});
     uint32_t g = s->n_geoms++;
+    // material index with the two-sided flag (bit 30) and, per triangle, the smooth flag (bit 29)
+    const int32_t mat_bits = mat | ((flags & TR_MATERIAL_TWO_SIDED) ? kMatTwoSided : 0);
+    if (normals && s->tri_vn.empty()) s->tri_vn.assign(s->tri_mat.size() * 9, 0.f);
     for (uint32_t t = 0; t < nt; ++t) {
         for (int k = 0; k < 3; ++k) {
             const float* p = xyz + 3 * size_t(idx[3 * t + k]);
             s->tris.insert(s->tris.end(), {p[0], p[1], p[2]});
         }
-        s->tri_mat.push_back(mat);
+        bool smooth = false;  // all-zero corner normals mean a flat triangle
+        if (normals)
+            for (int k = 0; k < 9; ++k) smooth = smooth || normals[9 * size_t(t) + k] != 0.f;
+        s->tri_mat.push_back(smooth ? (mat_bits | kMatSmooth) : mat_bits);
         s->tri_geom.push_back(g);
         s->tri_prim.push_back(t);
+        if (!s->tri_vn.empty()) {
+            if (normals) s->tri_vn.insert(s->tri_vn.end(), normals + 9 * size_t(t), normals + 9 * size_t(t) + 9);
+            else s->tri_vn.insert(s->tri_vn.end(), 9, 0.f);
+        }
     }
     if (geom_id) *geom_id = g;
     s->committed = false;
@@ -1318,6 +1376,24 @@ int tr_scene_commit(tr_scene s)
     if (!s->lights.empty())
         HIP_TRY(hipMemcpyAsync(s->d_lights, s->lights.data(), s->lights.size() * sizeof(LightHost), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_ids, ids.data(), ids.size() * sizeof(uint2), hipMemcpyHostToDevice, s->stream));
+    std::vector<float4> vn;
+    if (!s->tri_vn.empty()) {
+        // smooth triangles: the plane frame for barycentrics from a hit point, and corner normals
+        vn.resize(size_t(n) * 6);
+        for (uint32_t i = 0; i < n; ++i) {
+            const float* t = &s->tris[9 * size_t(i)];
+            const float* q = &s->tri_vn[9 * size_t(i)];
+            vn[6 * size_t(i) + 0] = make_float4(t[0], t[1], t[2], 0.f);
+            vn[6 * size_t(i) + 1] = make_float4(t[3] - t[0], t[4] - t[1], t[5] - t[2], 0.f);
+            vn[6 * size_t(i) + 2] = make_float4(t[6] - t[0], t[7] - t[1], t[8] - t[2], 0.f);
+            for (int k = 0; k < 3; ++k) vn[6 * size_t(i) + 3 + k] = make_float4(q[3 * k], q[3 * k + 1], q[3 * k + 2], 0.f);
+        }
+        if ((rc = dev_alloc(&s->d_tri_vn, vn.size(), &s->dev_bytes))) {
+            s->release_device();
+            return rc;
+        }
+        HIP_TRY(hipMemcpyAsync(s->d_tri_vn, vn.data(), vn.size() * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    }
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->n_nodes = uint32_t(nn);
     s->depth = b.depth;

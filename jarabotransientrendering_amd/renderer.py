@@ -81,21 +81,36 @@ class World:
     def handle(self):
         return self._h
 
-    def add_triangle_mesh(self, vertices: np.ndarray, indices: np.ndarray, material: Lambertian) -> int:
-        """World::add_triangle_mesh (World.h:209-541) from arrays: vertices (nv,3), indices (nt,3)."""
+    def add_triangle_mesh(self, vertices: np.ndarray, indices: np.ndarray, material: Lambertian,
+                          normals: Optional[np.ndarray] = None, two_sided: bool = False) -> int:
+        """World::add_triangle_mesh (World.h:209-541) from arrays: vertices (nv,3), indices (nt,3),
+        optional per-corner shading normals (nt,3,3) and the -two-sided material flag."""
         v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
         i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+        n = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 9)
         m = nat.Material((ctypes.c_float * 3)(*material.kd))
         gid = ctypes.c_uint32()
-        nat.check(nat.lib().tr_scene_add_mesh(self._h, nat.as_ptr(v), v.shape[0], nat.as_ptr(i), i.shape[0],
-                                              ctypes.byref(m), ctypes.byref(gid)))
+        nat.check(nat.lib().tr_scene_add_mesh_ex(self._h, nat.as_ptr(v), v.shape[0], nat.as_ptr(i), i.shape[0],
+                                                 nat.as_ptr(n) if n is not None else None, ctypes.byref(m),
+                                                 nat.TR_MATERIAL_TWO_SIDED if two_sided else 0, ctypes.byref(gid)))
         self.frozen = False
         return int(gid.value)
 
-    def add_triangles(self, tris: np.ndarray, material: Lambertian) -> int:
+    def add_obj(self, path: str, material: Lambertian, two_sided: bool = False) -> int:
+        """-name-mesh file.obj (World::add_triangle_mesh, World.h:209-541)."""
+        m = nat.Material((ctypes.c_float * 3)(*material.kd))
+        gid = ctypes.c_uint32()
+        nat.check(nat.lib().tr_scene_add_obj(self._h, path.encode(), ctypes.byref(m),
+                                             nat.TR_MATERIAL_TWO_SIDED if two_sided else 0, ctypes.byref(gid)))
+        self.frozen = False
+        return int(gid.value)
+
+    def add_triangles(self, tris: np.ndarray, material: Lambertian, normals: Optional[np.ndarray] = None,
+                      two_sided: bool = False) -> int:
         """Triangle soup (n,3,3) as one mesh."""
         t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 3)
-        return self.add_triangle_mesh(t, np.arange(t.shape[0], dtype=np.uint32).reshape(-1, 3), material)
+        return self.add_triangle_mesh(t, np.arange(t.shape[0], dtype=np.uint32).reshape(-1, 3), material,
+                                      normals, two_sided)
 
     def add_light(self, light) -> None:
         """World::add_light (World.h:179-182)."""
@@ -486,9 +501,15 @@ def setup_scene(scene):
     # one mesh per run of equal material, so scene triangle indices keep the scene's order
     tm = np.asarray(scene.tri_mat)
     start = 0
+    vn = getattr(scene, "vertex_normals", None)
+    ts = getattr(scene, "two_sided", None)
     for i in range(1, len(tm) + 1):
         if i == len(tm) or tm[i] != tm[start]:
-            w.add_triangles(scene.tris[start:i], Lambertian(scene.materials[tm[start]]))
+            nrm = None
+            if vn is not None and np.any(vn[start:i] != 0):
+                nrm = vn[start:i]
+            w.add_triangles(scene.tris[start:i], Lambertian(scene.materials[tm[start]]), nrm,
+                            bool(ts[tm[start]]) if ts is not None else False)
             start = i
     for p in scene.point_lights:
         w.add_light(PointLightSource(p[0:3], p[3:6]))

@@ -70,6 +70,8 @@ class Scene:
     point_lights: np.ndarray         # (p, 6) float32: position, RGB intensity
     rect_lights: np.ndarray          # (r, 9) float32: corner, target, size_x, size_y, intensity
     medium: Optional[np.ndarray] = None  # (6,) float32 sigma_a RGB, sigma_s RGB (isotropic phase)
+    vertex_normals: Optional[np.ndarray] = None  # (n, 3, 3) float32 corner normals, zeros = flat triangle
+    two_sided: Optional[np.ndarray] = None       # (m,) bool: material is two-sided (-two-sided)
     camera: Camera = dataclasses.field(default_factory=Camera)
     film: Film = dataclasses.field(default_factory=Film)
     render: RenderParams = dataclasses.field(default_factory=RenderParams)
@@ -92,8 +94,16 @@ class Scene:
         ]
         for m in self.materials:
             lines.append("material " + " ".join(repr(float(v)) for v in m))
-        for t, m in zip(self.tris, self.tri_mat):
-            lines.append("tri " + " ".join(repr(float(v)) for v in t.reshape(-1)) + f" {int(m)}")
+        vn = self.vertex_normals
+        for i, (t, m) in enumerate(zip(self.tris, self.tri_mat)):
+            if vn is not None and np.any(vn[i] != 0):
+                lines.append("trin " + " ".join(repr(float(v)) for v in t.reshape(-1)) + " " +
+                             " ".join(repr(float(v)) for v in vn[i].reshape(-1)) + f" {int(m)}")
+            else:
+                lines.append("tri " + " ".join(repr(float(v)) for v in t.reshape(-1)) + f" {int(m)}")
+        if self.two_sided is not None:
+            for m in np.flatnonzero(self.two_sided):
+                lines.append(f"twosided {int(m)}")
         for p in self.point_lights:
             lines.append("point " + " ".join(repr(float(v)) for v in p))
         for q in self.rect_lights:
@@ -304,3 +314,34 @@ def batch_scenes(n: int = 64, first_seed: int = 0) -> List[Scene]:
         s.name = f"batch{k}_ico{p['level']}"
         out.append(s)
     return out
+
+
+def smooth_scene(level=3, width=16, height=16, time_bins=256, exposure=0.01, sqrt_spp=2, n_incoming=1,
+                 max_bounces=6, seed=0) -> Scene:
+    """A smooth-shaded icosphere (corner normals = the unit sphere's normals, as an OBJ exporter
+    writes "vn" records) and a two-sided vertical panel lit from one side, in the open box with
+    the point light of the CPU-reference scene."""
+    tris, tri_mat, mats = box_shell(inner_box=False)
+    centre, radius = np.array([0.35, 0.3, 0.55]), 0.2
+    verts, faces = icosphere_mesh(level)
+    sph = (centre + radius * verts[faces]).astype(np.float32)
+    sph = np.array([_orient(t, None, centre) for t in sph], dtype=np.float32)
+    vn = np.zeros((tris.shape[0] + sph.shape[0] + 2, 3, 3), dtype=np.float32)
+    nrm = sph - centre.astype(np.float32)
+    vn[tris.shape[0]:tris.shape[0] + sph.shape[0]] = nrm / np.linalg.norm(nrm, axis=2, keepdims=True)
+    # two-sided panel in the plane x = 0.7, light at x = 0.5 on one side of it
+    panel = np.array([[[0.7, 0.05, 0.3], [0.7, 0.05, 0.8], [0.7, 0.6, 0.8]],
+                      [[0.7, 0.05, 0.3], [0.7, 0.6, 0.8], [0.7, 0.6, 0.3]]], dtype=np.float32)
+    mats = np.concatenate([mats, np.array([[0.7, 0.7, 0.7], [0.5, 0.6, 0.7]], dtype=np.float32)])
+    tri_all = np.concatenate([tris, sph, panel])
+    mat_all = np.concatenate([tri_mat, np.full(sph.shape[0], len(mats) - 2, np.int32),
+                              np.full(2, len(mats) - 1, np.int32)])
+    two = np.zeros(len(mats), dtype=bool)
+    two[-1] = True
+    return Scene(
+        tris=tri_all, tri_mat=mat_all.astype(np.int32), materials=mats,
+        point_lights=np.array([[0.5, 0.95, 0.5, 10.0, 10.0, 10.0]], dtype=np.float32),
+        rect_lights=np.zeros((0, 9), dtype=np.float32), vertex_normals=vn, two_sided=two,
+        film=Film(width, height, time_bins, exposure),
+        render=RenderParams(sqrt_spp, n_incoming, max_bounces, seed),
+        name=f"smooth{level}_box")

@@ -560,7 +560,27 @@ typedef struct {
     float time;
     int medium;   /* vertex in medium (scatter) */
     int mat;
+    int two_sided;
 } vertex_t;
+
+/* shading normal at a surface hit: the triangle's geometric normal, or for a smooth triangle its
+ * corner normals interpolated with the barycentrics of p and not renormalised (World.h:589-598,
+ * Triangle.h:155-159) */
+static v3 hit_normal(const world_t* w, int tri, v3 p)
+{
+    const float* q = w->s->vn ? w->s->vn + 9 * (size_t)tri : NULL;
+    int smooth = 0;
+    if (q)
+        for (int k = 0; k < 9; ++k) smooth |= q[k] != 0.0f;
+    if (!smooth) return w->tris[tri].shading_n;
+    const tri_t* T = &w->tris[tri];
+    v3 e1 = vsub(T->p1, T->p0), e2 = vsub(T->p2, T->p0), d = vsub(p, T->p0);
+    float d00 = vdot(e1, e1), d01 = vdot(e1, e2), d11 = vdot(e2, e2), d20 = vdot(d, e1), d21 = vdot(d, e2);
+    float inv = 1.0f / (d00 * d11 - d01 * d01);
+    float u = (d11 * d20 - d01 * d21) * inv, v = (d00 * d21 - d01 * d20) * inv;
+    v3 n0 = V(q[0], q[1], q[2]), n1 = V(q[3], q[4], q[5]), n2 = V(q[6], q[7], q[8]);
+    return vadd(vadd(vmul(n0, 1.0f - u - v), vmul(n1, u)), vmul(n2, v));
+}
 
 typedef struct { float rgb[3]; float time; int bounce; } rsample;
 
@@ -574,6 +594,8 @@ static void generate_path(ctx_t* c, rngs* R, v3 O, v3 D, int max_size, vertex_t*
     int level = 0;
     int cur_tri = -1;  /* surface hit of the previous iteration */
     v3 cur_pos = O;    /* position of the previous hit / scatter */
+    v3 cur_n = V(0, 0, 0), cur_din = V(0, 0, 0);  /* its shading normal and the ray that hit it */
+    int cur_two = 0;
     *nv = 0;
     for (;;) {
         v3 ut = V(0, 0, 0), us = V(0, 0, 0);
@@ -611,12 +633,14 @@ static void generate_path(ctx_t* c, rngs* R, v3 O, v3 D, int max_size, vertex_t*
             float e1 = rnd_path(R), e2 = rnd_path(R);
             float pdf;
             v3 l = sample_cosine(e1, e2, &pdf);
-            v3 n = c->w.tris[cur_tri].shading_n;
+            v3 n = cur_n;
             v3 aux = (fabsf(n.x) > fabsf(n.z)) ? V(-n.y, n.x, n.z) : V(0.0f, -n.z, n.y);
             v3 tv = vnorm(aux);
             v3 tu = vcross(n, tv);
             v3 nd = vadd(vadd(vmul(tu, l.x), vmul(n, l.y)), vmul(tv, l.z));
             nd = vnorm(nd);
+            /* two-sided material hit from behind: inverted frame (TwoSided.h:80-88, Intersection.h:180-185) */
+            if (cur_two && vdot(vneg(cur_din), n) < 0.0f) nd = vneg(nd);
             const float* kd = s->mats + 3 * (size_t)s->tri_mat[cur_tri];
             float inv_pi = (float)M_1_PI;
             fa = V(kd[0] * inv_pi, kd[1] * inv_pi, kd[2] * inv_pi);
@@ -650,13 +674,17 @@ static void generate_path(ctx_t* c, rngs* R, v3 O, v3 D, int max_size, vertex_t*
         v3 albedo;
         if (media_interaction) {
             cur_pos = vadd(O, vmul(D, sampled));
-            vx->medium = 1; vx->mat = -1; vx->normal = V(0, 0, 0);
+            vx->medium = 1; vx->mat = -1; vx->normal = V(0, 0, 0); vx->two_sided = 0;
             albedo = V(us.x / ut.x, us.y / ut.y, us.z / ut.z);
             cur_tri = -1;
         } else {
             cur_pos = vadd(O, vmul(D, sampled));
             vx->medium = 0; vx->mat = s->tri_mat[tri];
-            vx->normal = c->w.tris[tri].shading_n;
+            vx->normal = hit_normal(&c->w, tri, cur_pos);
+            vx->two_sided = s->mat_flags ? (s->mat_flags[vx->mat] & 1) : 0;
+            cur_n = vx->normal;
+            cur_two = vx->two_sided;
+            cur_din = D;
             const float* kd = s->mats + 3 * (size_t)vx->mat;
             /* 1 - get_absorption = 1 - (1 - Kd) (Lambertian.h:56-59, BidirectionalPathTracing.h:632) */
             albedo = V(1.0f - (1.0f - kd[0]), 1.0f - (1.0f - kd[1]), 1.0f - (1.0f - kd[2]));
@@ -699,7 +727,8 @@ static int connect_paths(ctx_t* c, rngs* R, const vertex_t* verts, int nv, int m
             float inv_pi = (float)M_1_PI;
             fe = V(kd[0] * inv_pi, kd[1] * inv_pi, kd[2] * inv_pi);
             float d = vdot(wl, v->normal);
-            fe = vmul(fe, d > 0.0f ? d : 0.0f);       /* dot_clamped (Path.h:188-193) */
+            /* dot_clamped, dot_abs for a two-sided material (Path.h:188-193) */
+            fe = vmul(fe, v->two_sided ? fabsf(d) : (d > 0.0f ? d : 0.0f));
         }
         v3 att = V(1, 1, 1);
         if (c->has_medium)                             /* compute_attenuation (Path.h:208-226) */

@@ -43,6 +43,11 @@ typedef struct {
     const float* rects;     /* n_rects*9: corner, target point, size x, size y, intensity */
     int32_t has_medium;
     float medium[6];        /* sigma_a RGB, sigma_s RGB; isotropic phase, no time delay */
+    /* optional: per-corner shading normals, 9 floats per triangle (all zero = flat triangle),
+     * interpolated at the hit point (World.h:589-598); per-material flags, bit 0 = two-sided
+     * (TwoSided.h:34-96) */
+    const float* vn;
+    const int32_t* mat_flags;
 } orc_scene;
 
 typedef struct {

@@ -28,6 +28,7 @@ class _Scene(ctypes.Structure):
         ("n_points", ctypes.c_int32), ("points", ctypes.c_void_p),
         ("n_rects", ctypes.c_int32), ("rects", ctypes.c_void_p),
         ("has_medium", ctypes.c_int32), ("medium", ctypes.c_float * 6),
+        ("vn", ctypes.c_void_p), ("mat_flags", ctypes.c_void_p),
     ]
 
 
@@ -131,6 +132,12 @@ def _scene_struct(scene):
         s.has_medium = 1
         for i in range(6):
             s.medium[i] = float(scene.medium[i])
+    vn = getattr(scene, "vertex_normals", None)
+    if vn is not None:
+        s.vn = arr(np.asarray(vn, dtype=np.float32).reshape(-1), np.float32)
+    ts = getattr(scene, "two_sided", None)
+    if ts is not None:
+        s.mat_flags = arr(np.asarray(ts, dtype=np.int32).reshape(-1), np.int32)
     return s, keep
 
 

@@ -36,6 +36,7 @@
 #include "bunnykiller.h"
 #undef _USE_EMBREE_
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -53,6 +54,7 @@
 #include "RayTracing/World.h"
 #include "Renderer/TransientRenderer.h"
 #include "Film/StreakCameraFilm.h"
+#include "Material/TwoSided.h"
 #include "Film/MultibounceStreakCameraFilm.h"
 #include "Filter/BoxFilter.h"
 #include "Geometry/Aggregate/Mesh.h"
@@ -80,6 +82,8 @@ struct SceneDesc {
     std::vector<std::array<double, 3>> materials;
     std::vector<std::array<double, 9>> tris;
     std::vector<int> tri_mat;
+    std::vector<std::array<double, 9>> tri_vn;  // corner normals ("trin"); all zero = flat
+    std::vector<int> two_sided;                  // material indices with TwoSidedMaterial
     std::vector<std::array<double, 6>> points;
     std::vector<std::array<double, 9>> rects;
     bool has_medium = false;
@@ -109,6 +113,12 @@ static SceneDesc parse_scene(const char* path)
         } else if (key == "tri") {
             std::array<double, 9> t; for (double& v : t) ls >> v;
             int m; ls >> m; s.tris.push_back(t); s.tri_mat.push_back(m);
+            s.tri_vn.push_back(std::array<double, 9>{});
+        } else if (key == "trin") {
+            std::array<double, 9> t, n; for (double& v : t) ls >> v; for (double& v : n) ls >> v;
+            int m; ls >> m; s.tris.push_back(t); s.tri_mat.push_back(m); s.tri_vn.push_back(n);
+        } else if (key == "twosided") {
+            int m; ls >> m; s.two_sided.push_back(m);
         } else if (key == "point") {
             std::array<double, 6> p; for (double& v : p) ls >> v; s.points.push_back(p);
         } else if (key == "rect") {
@@ -132,8 +142,16 @@ static std::vector<Triangle> make_triangles(const SceneDesc& s, std::vector<Mate
     for (size_t i = 0; i < s.tris.size(); ++i) {
         const auto& t = s.tris[i];
         Vector3 v0(t[0], t[1], t[2]), v1(t[3], t[4], t[5]), v2(t[6], t[7], t[8]);
-        // swapped: Triangle's normal = normalize(cross(b-a, c-a)) = normalize(cross(v2-v0, v1-v0))
-        tris.push_back(Triangle(v0, v2, v1, mats[s.tri_mat[i]]));
+        const auto& q = s.tri_vn[i];
+        bool smooth = false;
+        for (double x : q) smooth = smooth || x != 0.0;
+        // swapped: Triangle's normal = normalize(cross(b-a, c-a)) = normalize(cross(v2-v0, v1-v0));
+        // corner normals follow their vertices
+        if (smooth)
+            tris.push_back(Triangle(v0, Vector3(q[0], q[1], q[2]), v2, Vector3(q[6], q[7], q[8]), v1,
+                                    Vector3(q[3], q[4], q[5]), mats[s.tri_mat[i]]));
+        else
+            tris.push_back(Triangle(v0, v2, v1, mats[s.tri_mat[i]]));
     }
     return tris;
 }
@@ -198,10 +216,12 @@ static int run_render(const SceneDesc& s, bool reseed_per_camera_sample)
 {
     World<3, Radiance> w;
     std::vector<Material3D*> mats;
-    for (const auto& m : s.materials) {
+    for (size_t mi = 0; mi < s.materials.size(); ++mi) {
+        const auto& m = s.materials[mi];
         UberMaterial3D* um = new UberMaterial3D();
         um->add_bsdf(new Lambertian3D(Spectrum(m[0], m[1], m[2])));
-        mats.push_back(um);
+        bool two = std::find(s.two_sided.begin(), s.two_sided.end(), (int)mi) != s.two_sided.end();
+        mats.push_back(two ? static_cast<Material3D*>(new TwoSidedMaterial3D(um)) : static_cast<Material3D*>(um));
     }
     // one Mesh per material, triangles in scene order
     for (size_t mi = 0; mi < mats.size(); ++mi) {

@@ -68,6 +68,7 @@ def run_film(s, mode: str, tmp: str, tag: str) -> np.ndarray:
 def describe(s) -> dict:
     return {
         "scene": s.name, "n_tris": s.n_tris, "tris_sha256": scene_hash(s),
+        "smooth": s.vertex_normals is not None, "two_sided": None if s.two_sided is None else s.two_sided.tolist(),
         "film": vars(s.film), "camera": vars(s.camera), "render": vars(s.render),
         "medium": None if s.medium is None else [float(x) for x in s.medium],
         "point_lights": s.point_lights.tolist(), "rect_lights": s.rect_lights.tolist(),
@@ -86,8 +87,10 @@ def film_cases():
     rect.film.camera_unwarp = True
     rect.film.offset = 0.5
     rect.name = "box_rect_unwarp"
+    smooth = scenes.smooth_scene(level=3, width=16, height=16, time_bins=256, sqrt_spp=2, n_incoming=4,
+                                 max_bounces=6, seed=7)
     return [("box_global", box, "render"), ("media_cs", media, "render_cs"), ("ico3_cs", ico, "render_cs"),
-            ("rect_unwarp_cs", rect, "render_cs")]
+            ("rect_unwarp_cs", rect, "render_cs"), ("smooth_cs", smooth, "render_cs")]
 
 
 def stats_cases():
@@ -160,6 +163,20 @@ def main(only=None):
     subprocess.check_call(["sh", os.path.join(ROOT, "oracle", "build_ref.sh")])
     if only == "images":
         write_images()
+        return
+    if only is not None and only.startswith("film:"):
+        tag = only.split(":", 1)[1]
+        with open(os.path.join(HERE, "manifest.json")) as fh:
+            manifest = json.load(fh)
+        with tempfile.TemporaryDirectory() as tmp:
+            for t, sc, mode in film_cases():
+                if t == tag:
+                    film = run_film(sc, mode, tmp, t)
+                    np.savez_compressed(os.path.join(HERE, f"film_{t}.npz"), film=film)
+                    manifest["films"][t] = {"mode": mode, **describe(sc)}
+                    print(t, film.shape)
+        with open(os.path.join(HERE, "manifest.json"), "w") as fh:
+            json.dump(manifest, fh, indent=1, default=float)
         return
     if only == "multibounce":
         with open(os.path.join(HERE, "manifest.json")) as fh:

@@ -141,7 +141,10 @@ def _cases():
     mrect.point_lights = np.zeros((0, 6), np.float32)
     mrect.rect_lights = np.array([[0.65, 0.99, 0.65, 0.65, 0.0, 0.65, 0.3, 0.3, 20.0]], dtype=np.float32)
     mrect.film.camera_unwarp = True
-    return {"box": box, "media": media, "ico3": ico, "rect_unwarp_offset": rect, "media_rect_lastrow": mrect}
+    smooth = scenes.smooth_scene(level=3, width=16, height=16, time_bins=256, sqrt_spp=2, n_incoming=4,
+                                 max_bounces=6, seed=9)
+    return {"box": box, "media": media, "ico3": ico, "rect_unwarp_offset": rect, "media_rect_lastrow": mrect,
+            "smooth_two_sided": smooth}
 
 
 @pytest.mark.parametrize("name", list(_cases()))

@@ -53,6 +53,10 @@ def _scene_from_manifest(entry):
     elif name == "media_box":
         s = scenes.media_box(f["width"], f["height"], f["scanline"], f["time_bins"], f["exposure"], r["sqrt_spp"],
                              r["n_incoming"], r["max_bounces"], r["seed"])
+    elif name.startswith("smooth"):
+        level = int(name[len("smooth"):].split("_")[0])
+        s = scenes.smooth_scene(level, f["width"], f["height"], f["time_bins"], f["exposure"], r["sqrt_spp"],
+                                r["n_incoming"], r["max_bounces"], r["seed"])
     elif name.startswith("icosphere"):
         level = int(name[len("icosphere"):].split("_")[0])
         s = scenes.icosphere_box(level, f["width"], f["height"], f["time_bins"], f["exposure"], r["sqrt_spp"],
@@ -75,7 +79,7 @@ def _scene_from_manifest(entry):
 MODES = {"render": "RNG_REFERENCE", "render_cs": "RNG_REFERENCE_PER_CS"}
 
 
-@pytest.mark.parametrize("tag", ["box_global", "media_cs", "ico3_cs", "rect_unwarp_cs"])
+@pytest.mark.parametrize("tag", ["box_global", "media_cs", "ico3_cs", "rect_unwarp_cs", "smooth_cs"])
 def test_oracle_reproduces_reference_film(oracle, golden_manifest, tag):
     entry = golden_manifest["films"][tag]
     s = _scene_from_manifest(entry)
