@@ -87,6 +87,17 @@ public:
         frozen_ = false;
         return g;
     }
+    // World::add_triangle_mesh(name_file, mat) (World.h:209-541): a Wavefront OBJ file; a
+    // two-sided material is TwoSidedMaterial around the Lambertian (-two-sided, bunnykiller.cpp:408-412)
+    unsigned add_triangle_mesh(const std::string& obj_file, const Lambertian& m, bool two_sided = false)
+    {
+        tr_material mat{{m.Kd[0], m.Kd[1], m.Kd[2]}};
+        uint32_t g = 0;
+        check(tr_scene_add_obj(h_, obj_file.c_str(), &mat, two_sided ? TR_MATERIAL_TWO_SIDED : 0u, &g),
+              "add_triangle_mesh");
+        frozen_ = false;
+        return g;
+    }
     void add_light(const PointLightSource& l)
     {
         check(tr_scene_add_point_light(h_, l.position.data(), l.intensities.data()), "add_light");

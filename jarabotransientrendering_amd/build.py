@@ -25,6 +25,9 @@ FLAGS = [
 
 CLI_SRC = os.path.join(PKG, "cli", "render_scene.cpp")
 CLI = os.path.join(LIBDIR, "render_scene")
+# the reference's command line (bunnykiller.cpp) over the same library
+BK_SRC = os.path.join(PKG, "cli", "bunnykiller.cpp")
+BK = os.path.join(LIBDIR, "bunnykiller")
 
 
 def _run(cmd, verbose):
@@ -42,9 +45,10 @@ def build(verbose: bool = False) -> str:
           *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp], verbose)
     os.replace(tmp, LIB)
     # the C++ host driver over the C ABI (plain g++; finds the library next to itself)
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), CLI_SRC,
-          "-L", LIBDIR, "-ltransient_hip", "-Wl,-rpath,$ORIGIN", "-o", CLI + ".tmp"], verbose)
-    os.replace(CLI + ".tmp", CLI)
+    for src, exe in ((CLI_SRC, CLI), (BK_SRC, BK)):
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+              "-L", LIBDIR, "-ltransient_hip", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"], verbose)
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 

@@ -71,3 +71,83 @@ def test_driver_matches_python_host(tmp_path, kind):
     # same random streams and geometry order; only float-atomic summation order differs
     np.testing.assert_allclose(raw, img, rtol=1e-5, atol=1e-7)
     assert "Total time" in (tmp_path / "out.log").read_text()
+
+
+# ---------------------------------------------------------------------------------------------
+# the reference's command line (cli/bunnykiller.cpp)
+
+def _bk(args, cwd):
+    from jarabotransientrendering_amd.build import BK
+    assert os.path.exists(BK), "build the library and drivers first (__graft_entry__.build())"
+    return subprocess.run([BK, *args], cwd=cwd, capture_output=True, text=True, timeout=120)
+
+
+def _scene_args(s, tmp_path, name="out"):
+    """-name-mesh OBJ files per run of equal material plus the film/camera/integrator flags of `s`."""
+    args = []
+    tm = s.tri_mat
+    start = 0
+    for i in range(1, len(tm) + 1):
+        if i == len(tm) or tm[i] != tm[start]:
+            tri = s.tris[start:i].reshape(-1, 3)
+            lines = [f"v {float(x)!r} {float(y)!r} {float(z)!r}" for x, y, z in tri]
+            lines += [f"f {k} {k + 1} {k + 2}" for k in range(1, len(tri) + 1, 3)]
+            p = tmp_path / f"mesh{start}.obj"
+            p.write_text("\n".join(lines) + "\n")
+            kd = s.materials[tm[start]]
+            args += ["-name-mesh", str(p), "-lambertian", *[repr(float(x)) for x in kd]]
+            start = i
+    c, f, r = s.camera, s.film, s.render
+    args += ["-film-name", name, "-film-size-x", str(f.width), "-film-size-y", str(f.height),
+             "-film-size-t", str(f.time_bins), "-film-exposure", repr(float(f.exposure)),
+             "-camera-position", *[repr(float(x)) for x in c.position],
+             "-camera-focus", *[repr(float(x)) for x in c.focus], "-camera-up", *[repr(float(x)) for x in c.up],
+             "-camera-fov", repr(float(c.fov_deg)), "-camera-spp", str(r.sqrt_spp),
+             "-bidirectional-path-tracing", str(r.n_incoming), "-max-nb-bounces", str(r.max_bounces),
+             "-new-seed", str(r.seed)]
+    if f.scanline >= 0:
+        args += ["-film-scanline", str(f.scanline)]
+    if s.medium is not None:
+        args += ["-homogeneous-medium", *[repr(float(x)) for x in s.medium], "-pf-isotropic"]
+    for p in s.point_lights:
+        args += ["-point-light-source", *[repr(float(x)) for x in p]]
+    for q in s.rect_lights:  # last: the reference steps over the argument after it
+        args += ["-rectangular-area-light-source", *[repr(float(x)) for x in q]]
+    return args
+
+
+def test_bunnykiller_rejects_flags_outside_the_path(tmp_path):
+    # outside -name-mesh, a material flag is an unmatched argument and is skipped, as upstream
+    r = _bk(["-film-name", "x", "-phong", "1", "1", "1", "10", "-film-size-x", "4"], tmp_path)
+    assert "outside" not in r.stderr
+    r = _bk(["-name-mesh", "m.obj", "-phong", "1", "1", "1", "10"], tmp_path)
+    assert r.returncode == 1 and "-phong is outside" in r.stderr
+    r = _bk(["-time-sampling"], tmp_path)
+    assert r.returncode == 1 and "outside" in r.stderr
+    r = _bk(["-homogeneous-medium", "0", "0", "0", "1", "1", "1", "-pf-hg", "0.5"], tmp_path)
+    assert r.returncode == 1 and "-pf-hg" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["box", "media_rect"])
+def test_bunnykiller_command_line_matches_python_host(tmp_path, kind):
+    from jarabotransientrendering_amd import renderer
+    from tests.test_image_io import decode_rgbe
+    if kind == "box":
+        s = scenes.cornell_box(width=12, height=12, time_bins=64, exposure=0.04, sqrt_spp=2, n_incoming=3,
+                               max_bounces=4, seed=5)
+    else:
+        s = scenes.media_box(width=16, height=16, scanline=6, time_bins=128, exposure=0.02, sqrt_spp=2,
+                             n_incoming=2, max_bounces=6, seed=8)
+        s.point_lights = np.zeros((0, 6), np.float32)
+        s.rect_lights = np.array([[0.65, 0.99, 0.65, 0.65, 0.0, 0.65, 0.3, 0.3, 20.0]], dtype=np.float32)
+    r = _bk(_scene_args(s, tmp_path), tmp_path)
+    assert r.returncode == 0, r.stderr
+    img, _, _ = renderer.render_scene(s)
+    rows = [s.film.scanline] if s.film.scanline >= 0 else list(range(s.film.height))
+    for i, y in enumerate(rows):
+        q, f = decode_rgbe((tmp_path / f"out_{y:04d}.hdr").read_bytes())
+        m = img[i].max(axis=2, keepdims=True)
+        # same streams, RGBE quantisation (8 mantissa bits), float-atomic order
+        assert np.all(np.abs(f - img[i]) <= m / 100 + 1e-6 * img.max()), i
+    assert (tmp_path / "out.hdr").exists() and (tmp_path / "out.log").exists()
