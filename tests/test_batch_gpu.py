@@ -73,7 +73,7 @@ def test_graph_replay_equals_direct_launches(R):
     s = _jobs()[2]
     a, sa = _single(R, s, wave_paths=700)
     b, sb = _single(R, s, wave_paths=700, graph=True)
-    assert sa["waves"] == sb["waves"] > 3
+    assert sa["waves"] == sb["waves"] >= 3
     assert sa["kernel_launches"] == sb["kernel_launches"]
     assert sa["extension_rays"] == sb["extension_rays"]
     np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-7)
