@@ -138,7 +138,7 @@ def rel_mse(x, r, eps_scale=1e-2, trim=0.0):
 
 
 def guided_comparison(width=128, height=128, ref_spp=65536, pass_sqrt=8, checkpoints=(1, 2, 4, 8, 16, 32),
-                      guide_wave=1 << 18, steps_per_wave=4):
+                      guide_wave=1 << 20, steps_per_wave=8):
     from jarabotransientrendering_amd import renderer
     s = guided_scene(width, height)
     world, cam, sampler, film, integ = renderer.setup_scene(s)
@@ -151,6 +151,7 @@ def guided_comparison(width=128, height=128, ref_spp=65536, pass_sqrt=8, checkpo
     for k in range(n_ref):
         eng.render(None, world, integ, cam, film, sampler, seed=100000 + k)
     ref = film.read().astype(np.float64)
+    ref_steady = ref.sum(axis=2)  # time-integrated image
     ref_sec = time.perf_counter() - t0
     sampler.sqrt_spp = pass_sqrt
     pass_paths = s.n_paths() // (s.render.sqrt_spp ** 2) * pass_sqrt ** 2
@@ -166,7 +167,8 @@ def guided_comparison(width=128, height=128, ref_spp=65536, pass_sqrt=8, checkpo
                 done += 1
             img = film.read()
             out.append({"passes": cp, "spp": cp * pass_sqrt ** 2, "seconds": t_render,
-                        "rel_mse": rel_mse(img, ref), "rel_mse_trim": rel_mse(img, ref, trim=TRIM)})
+                        "rel_mse": rel_mse(img, ref), "rel_mse_trim": rel_mse(img, ref, trim=TRIM),
+                        "rel_mse_steady": rel_mse(img.sum(axis=2), ref_steady)})
         return out
 
     plain = run(None)
@@ -197,7 +199,8 @@ def guided_comparison(width=128, height=128, ref_spp=65536, pass_sqrt=8, checkpo
                      f"medium (sigma_a 0.05, sigma_s 0.5), 0.3x0.3 area light; {width}x{height} px x 1024 bins "
                      f"of 0.01; max 16 events; passes of {pass_sqrt ** 2} spp ({pass_paths} paths)"),
         "reference": {"spp": n_ref * 1024, "seconds": ref_sec, "estimator": "plain, orthonormal frame"},
-        "rel_mse_eps": "1e-2 * mean(reference)^2", "target_metric": "rel_mse_trim (largest 0.1 % of per-element errors left out)",
+        "rel_mse_eps": "1e-2 * mean(reference)^2", "rel_mse_steady": "relMSE of the time-integrated image",
+        "guide_wave_paths": guide_wave, "steps_per_wave": steps_per_wave, "target_metric": "rel_mse_trim (largest 0.1 % of per-element errors left out)",
         "target_rel_mse": target,
         "seconds_to_target": {"plain": t_plain, "guided": t_guided},
         "speedup": t_plain / t_guided,

@@ -203,7 +203,13 @@ enum {
     TR_RENDER_ORTHONORMAL_FRAME = 4u,
     /* capture the kernels of one wavefront (camera + every bounce) in a hipGraph once per call
      * and replay it for each wavefront; per-kernel timing is not available in this mode */
-    TR_RENDER_GRAPH = 8u
+    TR_RENDER_GRAPH = 8u,
+    /* the integrator's Transient sampling mode, -time-sampling (BidirectionalPathTracing.h:190-198,
+     * bunnykiller.cpp:1811-1817): free flights at rate 4 / (T * exposure), directions at medium
+     * vertices drawn by time-angular sampling towards a light point chosen per path
+     * (SamplingTechniques.h:170-250), no Russian roulette (paths end at the film's time length or
+     * the bounce cap). A per-job flag: jobs of one batch may differ. Not combined with guiding. */
+    TR_RENDER_TIME_SAMPLING = 16u
 };
 
 enum {

@@ -258,7 +258,8 @@ public:
     unsigned nb_bounces() const { return unsigned(d_.bounce_layers); }
 };
 
-// BidirectionalPathTracing in path-tracing-only SteadyState mode (BidirectionalPathTracing.h:122-217)
+// BidirectionalPathTracing in path-tracing-only mode (BidirectionalPathTracing.h:122-217) with the
+// SteadyState sampling techniques or the Transient ones (set_mode, :190-207)
 class BidirectionalPathTracing {
 public:
     enum SamplingMode { SteadyState, Transient };
@@ -273,19 +274,19 @@ public:
     {
         if (!on) throw std::runtime_error("BidirectionalPathTracing: light subpaths are not provided");
     }
-    void set_mode(SamplingMode m)
-    {
-        if (m != SteadyState) throw std::runtime_error("BidirectionalPathTracing: time sampling is not provided");
-    }
+    void set_mode(SamplingMode m) { mode_ = m; }
+    SamplingMode mode() const { return mode_; }
     void set_seed(uint64_t s) { seed_ = s; }
     void preprocess() const
     {
-        if (log_) {
-            std::fprintf(log_, "Bidirectional Path Tracing\n- Samples: %u\n- Next Event: Mean Free Path Sampling\n",
-                         n_);
-            std::fprintf(log_, "- Angular Sampling: Phase Function\n- Path Termination (Max Path Size %d): "
-                               "Russian Roulette\n- Vertices Connection: Shadow Connection\n\n", max_);
-        }
+        if (!log_) return;
+        const bool t = mode_ == Transient;
+        std::fprintf(log_, "Bidirectional Path Tracing\n- Samples: %u\n- Next Event: %s\n", n_,
+                     t ? "Time Sampling - #Vertices 4" : "Mean Free Path Sampling");
+        std::fprintf(log_, "- Angular Sampling: %s\n- Path Termination (Max Path Size %d): %s\n"
+                           "- Vertices Connection: %s\n\n",
+                     t ? "Shortest Time" : "Phase Function", max_, t ? "Total Time" : "Russian Roulette",
+                     t ? "Point-to-Line" : "Shadow Connection");
     }
     unsigned incoming_samples() const { return n_; }
     int max_path_size() const { return max_; }
@@ -297,15 +298,18 @@ private:
     FILE* log_;
     StreakCameraFilm* film_;
     int max_ = 30;  // constructor default (BidirectionalPathTracing.h:131)
+    SamplingMode mode_ = SteadyState;
     uint64_t seed_ = 0x853c49e6748fea9bULL;  // Random::RNG default state, as without -new-seed
 };
 
 // TransientRenderer (src/Renderer/TransientRenderer.h:77-201)
 class TransientRenderer {
 public:
+    // time_sampling only changes the time arguments World::Li passes on, which the path-tracing
+    // integrator ignores (TransientRenderer.h:160-164); the integrator's mode selects the techniques
     explicit TransientRenderer(FILE* f_log = nullptr, bool time_sampling = false) : log_(f_log)
     {
-        if (time_sampling) throw std::runtime_error("TransientRenderer: time sampling is not provided");
+        (void)time_sampling;
     }
     tr_render_stats render(const char* name, World& world, BidirectionalPathTracing* integrator,
                            const PinholeCamera& camera, StreakCameraFilm* film, StratifiedSampler* sampler) const
@@ -322,6 +326,7 @@ public:
         d.n_incoming = int32_t(integrator->incoming_samples());
         d.max_bounces = integrator->max_path_size();
         d.seed = integrator->seed();
+        if (integrator->mode() == BidirectionalPathTracing::Transient) d.flags |= TR_RENDER_TIME_SAMPLING;
         tr_render_stats st{};
         check(tr_render_transient(world.handle(), &cam, film->handle(), &d, &st), "TransientRenderer::render");
         if (log_) {

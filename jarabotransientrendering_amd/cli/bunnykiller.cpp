@@ -12,9 +12,9 @@
 //   -name-mesh file.obj [-lambertian r g b | -two-sided -lambertian r g b]  -lambertian-rho g
 //   -homogeneous-medium sa(3) ss(3) [-pf-isotropic] [-pf-time 0]
 //   -max-nb-bounces N  -bidirectional-path-tracing N  -new-seed S  -log-name F
-//   -transient-state  -standard-sampling  -scattering-level all  -parse-args-file F
+//   -transient-state  -standard-sampling  -time-sampling  -scattering-level all  -parse-args-file F
 // Flags of features outside the transient path-tracing path (other integrators, lights, BSDFs,
-// phase functions, time sampling, steady-state films) stop with an error instead of being
+// phase functions, steady-state films) stop with an error instead of being
 // ignored; unknown arguments are skipped, as the reference does (bunnykiller.cpp:1491-1494).
 // Output: "<name>_%04d.hdr" streak images and "<name>.hdr" (multibounce: "<name>_bXX_%04d.hdr").
 #include <cmath>
@@ -55,6 +55,7 @@ struct Params {
     std::vector<tr::RectangularAreaLightSource> rects;
     bool has_medium = false;
     tr::Vector3 sa{}, ss{};
+    bool time_sampling = false;
 };
 
 [[noreturn]] void unsupported(const std::string& flag)
@@ -173,11 +174,13 @@ void parse(Args& A, Params& p)
         else if (f == "-bidirectional-path-tracing") p.n_incoming = unsigned(A.n());
         else if (f == "-new-seed") p.seed = uint64_t(std::atoll(A.next().c_str()));
         else if (f == "-log-name") p.log_file = A.next();
-        else if (f == "-transient-state" || f == "-standard-sampling") {
-        } else if (f == "-scattering-level") {
+        else if (f == "-transient-state") {
+        } else if (f == "-standard-sampling") p.time_sampling = false;
+        else if (f == "-time-sampling") p.time_sampling = true;
+        else if (f == "-scattering-level") {
             if (A.next() != "all") unsupported("-scattering-level other than all");
         } else if (f == "-parse-args-file") parse_file(A.next(), p);
-        else if (f == "-steady-state" || f == "-time-sampling" || f == "-photon-mapping" || f == "-sphere" ||
+        else if (f == "-steady-state" || f == "-photon-mapping" || f == "-sphere" ||
                  f == "-plane" || f == "-mie-medium" || f == "-cosine-light-source" ||
                  f == "-hemispherical-light-source" || f == "-point-light-source-polarized" ||
                  f == "-spot-light-source" || f == "-directional-light-source" ||
@@ -237,9 +240,10 @@ int main(int argc, char** argv)
         std::printf("Rendering with Bidirectional Path Tracing...\n");
         bpt.set_max_path_size(p.max_bounces);
         bpt.set_path_tracing_only(true);
-        bpt.set_mode(tr::BidirectionalPathTracing::SteadyState);
+        // bunnykiller.cpp:1811-1817
+        bpt.set_mode(p.time_sampling ? tr::BidirectionalPathTracing::Transient : tr::BidirectionalPathTracing::SteadyState);
         bpt.set_seed(p.seed);
-        tr::TransientRenderer engine(f_log, false);
+        tr::TransientRenderer engine(f_log, p.time_sampling);
         tr_render_stats st = engine.render(p.image_file.c_str(), w, &bpt, camera, film.get(), &sampler);
         if (f_log) std::fclose(f_log);
         std::printf("{\"paths\": %llu, \"render_ms\": %.3f}\n", (unsigned long long)st.paths, st.render_ms);

@@ -12,6 +12,7 @@
 //   rect cx cy cz tx ty tz size_x size_y intensity
 //   medium sigma_a(rgb) sigma_s(rgb)
 //   multibounce nb_bounces
+//   timesampling            (the integrator's Transient sampling mode, -time-sampling)
 //   out name_prefix
 // Output: <prefix>_%04d.raw streak slices, <prefix>.log, and one JSON line of render statistics.
 #include <cstdio>
@@ -45,6 +46,7 @@ int main(int argc, char** argv)
         bool has_medium = false;
         tr::Vector3 sa{}, ss{};
         int multibounce = 0;
+        bool time_sampling = false;
         std::string out = "film";
         std::string line;
         // consecutive triangles of one material form one mesh, as a -name-mesh per material
@@ -89,6 +91,8 @@ int main(int argc, char** argv)
                 has_medium = true;
             } else if (key == "multibounce") {
                 ls >> multibounce;
+            } else if (key == "timesampling") {
+                time_sampling = true;
             } else if (key == "out") {
                 ls >> out;
             } else {
@@ -128,9 +132,9 @@ int main(int argc, char** argv)
         tr::BidirectionalPathTracing bpt(w, unsigned(n_inc), f_log, &film);
         bpt.set_max_path_size(max_b);
         bpt.set_path_tracing_only(true);
-        bpt.set_mode(tr::BidirectionalPathTracing::SteadyState);
+        bpt.set_mode(time_sampling ? tr::BidirectionalPathTracing::Transient : tr::BidirectionalPathTracing::SteadyState);
         bpt.set_seed(seed);
-        tr::TransientRenderer engine(f_log, false);
+        tr::TransientRenderer engine(f_log, time_sampling);
         tr_render_stats st = engine.render(out.c_str(), w, &bpt, camera, &film, &sampler);
         if (f_log) std::fclose(f_log);
         std::printf("{\"paths\": %llu, \"extension_rays\": %llu, \"shadow_rays\": %llu, \"contributions\": %llu, "

@@ -104,6 +104,9 @@ struct RenderDev {
     float* steady;             // [rows][W][3] every sample, any time (multibounce films)
     float* layer_w;            // [layers][rows][W] camera samples with samples of each layer
     uint32_t* cs_bits;         // per camera sample of the wavefront: layers that received samples
+    // Transient sampling mode (-time-sampling): free-flight rate 4 / (T * exposure); 0 = steady state
+    float ts_lambda;
+    int32_t time_sampling;
 };
 
 struct WaveDev {
@@ -148,6 +151,23 @@ __global__ void set_wave_kernel(WaveCtl* ctl, uint64_t base, uint32_t n)
 }
 
 __device__ __forceinline__ uint64_t u64(uint32_t lo, uint32_t hi) { return uint64_t(lo) | (uint64_t(hi) << 32); }
+
+// Free-flight distance -log(eps) / lambda (BidirectionalPathTracing.h:258-259, float log): the
+// log is taken in double and rounded, which gives the correctly rounded float log that host libms
+// return, so that a path's distances match the host's bit for bit in all but rare cases.
+__device__ __forceinline__ float free_flight(float eps, float lambda)
+{
+    return -float(log(double(eps))) / lambda;
+}
+
+// o + d * t rounded as two operations (no fused multiply-add), as the reference computes vertex
+// positions (Ray::get_position): the time-angular sampler's geometry is ill-conditioned near
+// backward scattering, so positions must match the host's to the last bit there.
+__device__ __forceinline__ f3 point_on_ray(f3 o, f3 d, float t)
+{
+#pragma clang fp contract(off)
+    return mk(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+}
 
 // ---------------------------------------------------------------------------------------------
 // camera: one thread per path of the wavefront
@@ -212,8 +232,7 @@ __global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S0, RenderDev R
     uint64_t nst = pcg_seed(R.seed, 2 * pid + 1);
     float sampled = __int_as_float(0x7f800000);
     if (S.has_medium) {
-        float eps = pcg_float(pst, pinc);
-        sampled = -logf(eps) / S.lambda;
+        sampled = free_flight(pcg_float(pst, pinc), R.time_sampling ? R.ts_lambda : S.lambda);
     }
     Wv.ray_o[slot] = make_float4(R.cpos.x, R.cpos.y, R.cpos.z, sampled);
     Wv.ray_d[slot] = make_float4(dir.x, dir.y, dir.z, 0.0f);
@@ -310,17 +329,11 @@ __device__ __forceinline__ f3 smooth_normal(const float4* V, f3 p)
     return mk(V[3]) * (1.0f - u - v) + mk(V[4]) * u + mk(V[5]) * v;
 }
 
-// Isotropic phase-function direction: uniform sphere rotated so that +y maps onto wi
-// (Sampling.h:80-94, IsotropicPhaseFunction.h:54-60, VectorN.h:429-523).
-__device__ __forceinline__ f3 sample_isotropic(f3 wi, float e1, float e2)
+// tVectorN::transform_matrix_to(from = (0,1,0), to) applied to v (VectorN.h:429-523): the
+// rotation taking +y onto `to`, built from `to` as given (the time-angular sampler passes an
+// unnormalised vector, as the reference does), result normalised.
+__device__ __forceinline__ f3 rotate_from_y(f3 v, f3 wi)
 {
-    float phi = 2.0f * kPi * e1;
-    float ct = 1.0f - 2.0f * e2;
-    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
-    float sp, cp;
-    sincosf(phi, &sp, &cp);
-    f3 v = mk(cp * st, ct, sp * st);
-    // transform_matrix_to(from = (0,1,0), to = wi)
     f3 from = mk(0.0f, 1.0f, 0.0f);
     f3 vx = cross(from, wi);
     float e = wi.y;
@@ -353,7 +366,80 @@ __device__ __forceinline__ f3 sample_isotropic(f3 wi, float e1, float e2)
     }
     f3 s = mk(m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z, m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
               m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z);
-    return normalize(normalize(s));
+    return normalize(s);
+}
+
+// Isotropic phase-function direction: uniform sphere rotated so that +y maps onto wi
+// (Sampling.h:80-94, IsotropicPhaseFunction.h:54-60); the new Ray normalises it again.
+__device__ __forceinline__ f3 sample_isotropic(f3 wi, float e1, float e2)
+{
+    float phi = 2.0f * kPi * e1;
+    float ct = 1.0f - 2.0f * e2;
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    return normalize(rotate_from_y(mk(cp * st, ct, sp * st), wi));
+}
+
+// Time-angular direction sampling at a medium vertex (AngularSampling::ShortestTime,
+// BidirectionalPathTracing.h:357-398 with Probability::TimeAngular, SamplingTechniques.h:170-250;
+// eta = 1, c = 1/time_of_flight(1) = 1): the angle theta to the direction of the light point `l`
+// is drawn so that a free flight of r followed by the straight line to the light arrives
+// uniformly in the time window [ta, tb] (clipped to what the geometry allows), ta being the
+// straight-line arrival time. The caller first checks time_angular_window (no draw): when the window
+// left after this flight is shorter than one exposure it samples the phase function. Returns false
+// when the density falls below 1e-5 (the caller then samples the phase function with two more
+// draws). pdf is per (theta, phi), as the reference uses it.
+__device__ __forceinline__ bool time_angular_window(float modl, float time, float r, float tb, float exposure)
+{
+#pragma clang fp contract(off)
+    const float ta = modl * 1.0f + time;
+    return !((tb - (ta + r * 1.0f)) < exposure);
+}
+
+__device__ __forceinline__ bool sample_time_angular(f3 l, float modl, float time, float r, float tb, float e_theta,
+                                                    float e_phi, f3* dir, float* pdf)
+{
+#pragma clang fp contract(off)
+    const float ta = modl * 1.0f + time;
+    // set_temporal_boundaries with ec = 1
+    float t_a = r > modl ? fmaxf(ta, 2.0f * r - modl) : fmaxf(ta, modl);
+    float t_b = fminf(tb, 2.0f * r + modl);
+    float t = e_theta * (t_b - t_a) + t_a;
+    float cos_t = float((double(modl * modl) + 2.0 * double(t) * double(r) - double(t * t)) /
+                        (2.0 * double(r) * double(modl)));
+    cos_t = fmaxf(-1.0f, fminf(cos_t, 1.0f));
+    const float theta = float(acos(double(cos_t)));  // correctly rounded, as the host's acosf
+    const float phi = float(2.0 * 3.14159265358979323846 * double(e_phi));
+    float sp, cp, st, ct;
+    sincosf(phi, &sp, &cp);
+    sincosf(theta, &st, &ct);
+    *dir = normalize(rotate_from_y(mk(cp * st, ct, sp * st), l));  // the new Ray normalises
+    const double den = sqrt(double(r * r + modl * modl) - 2.0 * double(r) * double(modl) * double(ct));
+    const float dtdtheta = float(double(modl * r * st) / den);
+    *pdf = float(double(dtdtheta / (t_b - t_a)) / (2.0 * 3.14159265358979323846));
+    return !(double(*pdf) < 1e-5);
+}
+
+// The light point a path aims at in the Transient sampling mode: the reference draws it per
+// incoming sample with World::sample_light + LightSource::sample(ls, pdf)
+// (BidirectionalPathTracing.h:1115-1118; PointLightSource.h:173-183, RectangularAreaLightSource.h:
+// 250-271); here it comes from the path's own stream seq = 3 * 2^61 | path id.
+__device__ __forceinline__ f3 light_target(const SceneDev& S, uint64_t seed, uint64_t pid)
+{
+#pragma clang fp contract(off)
+    const uint64_t seq = (3ull << 61) | pid;
+    uint64_t tinc = pcg_inc(seq), tst = pcg_seed(seed, seq);
+    float u = pcg_float(tst, tinc);
+    int li = min(int(u * float(S.n_lights)), S.n_lights - 1);
+    const LightHost& L = S.lights[li];
+    f3 p = mk(L.pos[0], L.pos[1], L.pos[2]);
+    if (int(L.I[3]) != kLightPoint) {
+        float a = pcg_float(tst, tinc), b = pcg_float(tst, tinc);
+        p = mk((p.x + L.left[0] * a) + L.up[0] * b, (p.y + L.left[1] * a) + L.up[1] * b,
+               (p.z + L.left[2] * a) + L.up[2] * b);
+    }
+    return p;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -421,16 +507,17 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
             // correct_next_event_distance (BidirectionalPathTracing.h:275-307)
             float dist, pd = 1.0f;
             bool alive = true;
+            const float lam = R.time_sampling ? R.ts_lambda : S.lambda;  // NextEventSampling::Time / MeanFreePath
             if (!S.has_medium) {
                 alive = tri >= 0;
                 dist = h.x;
             } else if (tri >= 0 && h.x < sampled) {
                 dist = h.x;
-                pd = expf(-dist * S.lambda);
+                pd = expf(-dist * lam);
             } else {
                 dist = sampled;
                 medium_event = true;
-                pd = expf(-dist * S.lambda) * S.lambda;
+                pd = expf(-dist * lam) * lam;
             }
             if (alive) {
                 f = mk(thr);
@@ -439,7 +526,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                     f = f * (1.0f / pd);
                 }
                 time = thr.w + dist;
-                pos = o + d * dist;
+                pos = point_on_ray(o, d, dist);
                 n = mk(0.0f, 0.0f, 0.0f);
                 if (!medium_event) {
                     float4 nn = S.tri_normal[tri];
@@ -535,18 +622,16 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                     }
                 }
                 // ---- termination: later vertices cannot contribute past the bounce cap or the
-                //      film window; otherwise Russian roulette on the mean albedo (:447-481)
+                //      film window; otherwise Russian roulette on the mean albedo, or in the
+                //      Transient mode the film's time length alone (TotalTime) (:447-481)
                 if (int(level) + 1 < R.max_bounces && time <= R.total_time) {
                     pst = u64(misc.z, misc.w);
                     float albedo = medium_event ? S.medium_albedo : S.mats[mat].w;
-                    float eps = pcg_float(pst, pinc);
+                    float eps = R.time_sampling ? 0.0f : pcg_float(pst, pinc);
                     if (!(eps > albedo)) {
-                        pi = albedo;
+                        pi = R.time_sampling ? 1.0f : albedo;
                         nsampled = __int_as_float(0x7f800000);
-                        if (S.has_medium) {
-                            float e = pcg_float(pst, pinc);
-                            nsampled = -logf(e) / S.lambda;
-                        }
+                        if (S.has_medium) nsampled = free_flight(pcg_float(pst, pinc), lam);
                         want = true;
                         if (GUIDED && tag != 0xFFFFFFFFu) tag += level;
                     }
@@ -587,10 +672,27 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
             bool ok = true;
             if constexpr (!GUIDED) {
                 if (medium_event) {
-                    float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
-                    nd = sample_isotropic(d, e1, e2);
+                    bool timed = false;
+                    if (R.time_sampling) {
+                        const f3 tg = light_target(S, R.seed, u64(misc.x, misc.y));
+                        f3 l;
+                        float modl;
+                        {
+#pragma clang fp contract(off)
+                            l = mk(tg.x - pos.x, tg.y - pos.y, tg.z - pos.z);
+                            modl = sqrtf((l.x * l.x + l.y * l.y) + l.z * l.z);
+                        }
+                        if (time_angular_window(modl, time, nsampled, R.total_time, R.exposure)) {
+                            float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
+                            timed = sample_time_angular(l, modl, time, nsampled, R.total_time, e1, e2, &nd, &pa);
+                        }
+                    }
+                    if (!timed) {
+                        float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
+                        nd = sample_isotropic(d, e1, e2);
+                        pa = kInv4Pi;
+                    }
                     fa = mk(S.sigma_s.x, S.sigma_s.y, S.sigma_s.z) * kInv4Pi;
-                    pa = kInv4Pi;
                 } else {
                     (void)pcg_float(pst, pinc);  // BSDF pick of a single-BSDF UberMaterial
                     float e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
@@ -1732,6 +1834,10 @@ int make_job(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* 
     R.exposure = fd.exposure;
     R.offset = fd.offset;
     R.total_time = float(fd.time_bins) * fd.exposure + fd.offset;
+    // NextEventSampling::Time: m_avg_nb_vertices_path (4) / (T * exposure / time_of_flight(1) / ior)
+    // (BidirectionalPathTracing.h:131, 246-252; World.h:776-786: time_of_flight(d) = d, ior = 1)
+    R.time_sampling = (d->flags & TR_RENDER_TIME_SAMPLING) ? 1 : 0;
+    R.ts_lambda = 4.0f / (float(fd.time_bins) * fd.exposure / 1.0f / 1.0f);
     R.inv_inc = float(1. / double(float(d->n_incoming)));
     R.unwarp = fd.camera_unwarp ? 1 : 0;
     R.max_bounces = d->max_bounces;
@@ -2086,6 +2192,8 @@ int tr_render_transient_guided(tr_scene s, const tr_camera* cam, tr_film f, cons
                                tr_render_stats* stats)
 {
     if (!g) return fail(TR_ERR_INVALID, "tr_render_transient_guided: null guide");
+    if (d && (d->flags & TR_RENDER_TIME_SAMPLING))
+        return fail(TR_ERR_INVALID, "tr_render_transient_guided: guiding does not combine with TR_RENDER_TIME_SAMPLING");
     return render_impl(s, cam, f, d, g, stats);
 }
 
@@ -2137,13 +2245,14 @@ LaunchCfg launch_cfg(const tr_batch_s* b, const tr_launch_desc* L)
     return c;
 }
 
-int batch_jobs(tr_batch b, std::vector<HostJob>* jobs, int sqrt_spp_override)
+// sqrt_spp_cap > 0 lowers every job's sqrt_spp to at most the cap (autotuner trials)
+int batch_jobs(tr_batch b, std::vector<HostJob>* jobs, int sqrt_spp_cap)
 {
     jobs->resize(b->entries.size());
     for (size_t k = 0; k < b->entries.size(); ++k) {
         const auto& e = b->entries[k];
         tr_render_desc d = e.d;
-        if (sqrt_spp_override > 0) d.sqrt_spp = sqrt_spp_override;
+        if (sqrt_spp_cap > 0) d.sqrt_spp = std::min(d.sqrt_spp, sqrt_spp_cap);
         int rc = make_job(e.s, &e.cam, e.f, &d, &(*jobs)[k], "tr_batch_render");
         if (rc) return rc;
     }
@@ -2215,12 +2324,26 @@ int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
     if (!b) return fail(TR_ERR_INVALID, "tr_autotune: null batch");
     if (b->entries.empty()) return fail(TR_ERR_INVALID, "tr_autotune: no job");
     std::vector<HostJob> jobs;
-    int rc = batch_jobs(b, &jobs, 1);
+    int rc = batch_jobs(b, &jobs, 0);
     if (rc) return rc;
     uint64_t total = 0;
+    int max_spp = 1;
     for (auto& j : jobs) total += j.paths;
+    for (auto& e : b->entries) max_spp = std::max(max_spp, e.d.sqrt_spp);
+    // the trials render the jobs with sqrt_spp capped so that a trial holds about as many paths as
+    // the largest candidate wavefront (or the whole batch, when it is smaller): the wavefront size
+    // that wins depends on how many paths are in flight
+    const uint64_t kMaxWave = 1ull << 25;
+    const uint64_t target = std::min<uint64_t>(total, kMaxWave);
+    uint64_t trial_total = 0;
+    for (int cap = 1; cap <= max_spp; ++cap) {
+        if ((rc = batch_jobs(b, &jobs, cap))) return rc;
+        trial_total = 0;
+        for (auto& j : jobs) trial_total += j.paths;
+        if (trial_total >= target) break;
+    }
     LaunchCfg base = b->tuned ? b->tune : LaunchCfg{};
-    // one trial: the jobs at one camera sample per pixel, best of two runs
+    // one trial: the capped jobs, best of two runs
     auto trial = [&](const LaunchCfg& c, double* ms) -> int {
         double m = 1e300;
         for (int rep = 0; rep < 2; ++rep) {
@@ -2237,8 +2360,8 @@ int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
     double best_ms = 1e300;
     // stage 1: wavefront size and graph replay; stage 2: persistent grid shapes
     std::vector<uint32_t> waves;
-    for (uint32_t w : {1u << 20, 1u << 22, 1u << 24, 1u << 25})
-        if (w <= std::max<uint64_t>(total, 1u << 20)) waves.push_back(w);
+    for (uint32_t w : {1u << 18, 1u << 20, 1u << 22, 1u << 24, 1u << 25})
+        if (w == (1u << 18) || w / 2 < trial_total) waves.push_back(w);
     for (uint32_t w : waves)
         for (bool g : {false, true}) {
             LaunchCfg c = base;

@@ -308,8 +308,9 @@ class MultibounceStreakFilm(StreakCameraFilm):
 
 
 class BidirectionalPathTracing:
-    """Integrator settings of BidirectionalPathTracing in its default path-tracing-only,
-    SteadyState-sampling configuration (BidirectionalPathTracing.h:122-217)."""
+    """Integrator settings of BidirectionalPathTracing in its path-tracing-only configuration
+    (BidirectionalPathTracing.h:122-217), with the SteadyState sampling techniques (the default) or
+    the Transient ones (set_mode(Transient), -time-sampling; :190-198)."""
 
     SteadyState, Transient = 0, 1
 
@@ -332,9 +333,12 @@ class BidirectionalPathTracing:
         self.path_trace_only = True
 
     def set_mode(self, mode: int) -> None:
-        if mode != self.SteadyState:
-            raise NotImplementedError("transient time-sampling mode is not part of this path")
+        if mode not in (self.SteadyState, self.Transient):
+            raise ValueError(f"unknown sampling mode {mode}")
         self.mode = mode
+
+    def _flags(self) -> int:
+        return nat.TR_RENDER_TIME_SAMPLING if self.mode == self.Transient else 0
 
     def preprocess(self) -> None:
         pass
@@ -404,8 +408,9 @@ class TransientRenderer:
     """TransientRenderer::render (src/Renderer/TransientRenderer.h:77-201)."""
 
     def __init__(self, f_log=None, time_sampling: bool = False):
-        if time_sampling:
-            raise NotImplementedError("transient time-sampling mode is not part of this path")
+        # time_sampling only changes the (unused) time arguments of World::Li
+        # (TransientRenderer.h:160-164); the integrator's mode selects the sampling techniques
+        self.time_sampling = bool(time_sampling)
         self.wave_paths = 0
         self.profile = False
         self.binary_bvh = False   # traverse the binary BVH instead of the 4-wide one
@@ -427,7 +432,7 @@ class TransientRenderer:
         integrator.preprocess()
         flags = ((nat.TR_RENDER_PROFILE if self.profile else 0) | (nat.TR_RENDER_BINARY_BVH if self.binary_bvh else 0)
                  | (nat.TR_RENDER_ORTHONORMAL_FRAME if self.orthonormal_frame else 0)
-                 | (nat.TR_RENDER_GRAPH if self.graph else 0))
+                 | (nat.TR_RENDER_GRAPH if self.graph else 0) | integrator._flags())
         d = nat.RenderDesc(sampler.sqrt_spp, integrator.incoming_samples, integrator.max_path_size, 0, int(seed),
                            int(self.wave_paths), flags)
         st = nat.RenderStats()
@@ -468,7 +473,8 @@ class Batch:
             world.freeze()
         if sampler.scanline >= 0 and film.streak != sampler.scanline:
             film.set_streak(sampler.scanline)
-        d = nat.RenderDesc(sampler.sqrt_spp, integrator.incoming_samples, integrator.max_path_size, 0, int(seed), 0, 0)
+        d = nat.RenderDesc(sampler.sqrt_spp, integrator.incoming_samples, integrator.max_path_size, 0, int(seed), 0,
+                           integrator._flags())
         cam = camera.desc()
         job = ctypes.c_uint32()
         nat.check(nat.lib().tr_batch_add(self.handle, world.handle, ctypes.byref(cam), film.handle, ctypes.byref(d),
@@ -533,6 +539,8 @@ def setup_scene(scene):
         film.set_streak(f.scanline)
     integ = BidirectionalPathTracing(w, scene.render.n_incoming, None, film)
     integ.set_max_path_size(scene.render.max_bounces)
+    if getattr(scene.render, "time_sampling", False):
+        integ.set_mode(BidirectionalPathTracing.Transient)
     return w, cam, sampler, film, integ
 
 

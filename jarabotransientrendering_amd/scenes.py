@@ -55,11 +55,12 @@ class Camera:
 @dataclasses.dataclass
 class RenderParams:
     """Sampler/integrator settings (-camera-spp, -bidirectional-path-tracing,
-    -max-nb-bounces, -new-seed; bunnykiller.cpp:1151, 1389, 1367, 1356)."""
+    -max-nb-bounces, -new-seed, -time-sampling; bunnykiller.cpp:1151, 1389, 1367, 1356, 1337-1340)."""
     sqrt_spp: int = 4
     n_incoming: int = 1
     max_bounces: int = 4
     seed: int = 0
+    time_sampling: bool = False  # the integrator's Transient sampling mode (BidirectionalPathTracing.h:190-198)
 
 
 @dataclasses.dataclass
@@ -112,6 +113,8 @@ class Scene:
             lines.append("medium " + " ".join(repr(float(v)) for v in self.medium))
         if f.bounce_layers:
             lines.append(f"multibounce {int(f.bounce_layers)}")
+        if r.time_sampling:
+            lines.append("timesampling")
         lines.append(f"out {out_prefix}")
         return "\n".join(lines) + "\n"
 

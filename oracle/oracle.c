@@ -149,13 +149,17 @@ static v3 transform_matrix_to(v3 v, v3 from, v3 to)
 
 typedef struct {
     int mode;
-    orc_pcg32* global;   /* ORC_RNG_REFERENCE */
-    orc_pcg32 path, nee; /* ORC_RNG_PER_PATH */
+    orc_pcg32* global;        /* ORC_RNG_REFERENCE */
+    orc_pcg32 path, nee, tgt; /* ORC_RNG_PER_PATH (tgt: the time-sampling light point) */
 } rngs;
 
 static inline float rnd_path(rngs* r)
 {
     return r->mode == ORC_RNG_REFERENCE ? orc_pcg32_next_float(r->global) : orc_pcg32_next_float(&r->path);
+}
+static inline float rnd_tgt(rngs* r)
+{
+    return r->mode == ORC_RNG_REFERENCE ? orc_pcg32_next_float(r->global) : orc_pcg32_next_float(&r->tgt);
 }
 static inline float rnd_nee(rngs* r)
 {
@@ -494,6 +498,9 @@ typedef struct {
     v3 sa, ss, st;
     float lambda;     /* mean extinction (HomogeneousMedium.h:45-58) */
     float total_time; /* T * exposure + offset (StreakCameraFilm.h:197-200) */
+    int time_sampling;
+    float ts_lambda;  /* NextEventSampling::Time: 4 / (T * exposure) (BidirectionalPathTracing.h:246-252) */
+    float exposure;
 } ctx_t;
 
 /* World::intersects (World.h:617-639): occlusion in (tnear, dist - eps] */
@@ -540,15 +547,62 @@ static int light_sample(ctx_t* c, rngs* R, int li, v3 p, v3* irr, float* dist, v
     return 0;
 }
 
-/* the reference draws a light-path origin even in path-tracing-only mode
- * (BidirectionalPathTracing.h:1117-1118): pick (1 draw) + point: spherical (2) / rect: uv (2) + cosine (2) */
-static void unused_light_path_draws(ctx_t* c, rngs* R)
+/* the reference draws a light-path origin for every incoming sample in path-tracing-only mode
+ * (BidirectionalPathTracing.h:1117-1118): World::sample_light's pick (1 draw), then
+ * PointLightSource::sample(ls, pdf) (PointLightSource.h:173-183: the light's position, a spherical
+ * direction, 2 draws) or RectangularAreaLightSource::sample(ls, pdf) (RectangularAreaLightSource.h:
+ * 250-271: a uniform point m_position + m_left*u + m_up*v, then a cosine direction, 2 draws).
+ * Only the position is used, as the target of time sampling (ShortestTime); in steady-state mode
+ * the draws are consumed and nothing is used. */
+static v3 light_path_origin(ctx_t* c, rngs* R)
 {
-    if (R->mode != ORC_RNG_REFERENCE) return;
-    float u = rnd_path(R);
+    v3 target = V(0, 0, 0);
+    if (R->mode != ORC_RNG_REFERENCE && !c->time_sampling) return target;
+    float u = rnd_tgt(R);
     int li = (int)(u * (float)c->n_lights);
-    rnd_unused(R); rnd_unused(R);
-    if (li >= c->w.s->n_points) { rnd_unused(R); rnd_unused(R); }
+    if (li >= c->n_lights) li = c->n_lights - 1;
+    if (li < c->w.s->n_points) {
+        const float* L = c->w.s->points + 6 * (size_t)li;
+        target = V(L[0], L[1], L[2]);
+        rnd_unused(R); rnd_unused(R);
+    } else {
+        const rect_t* L = &c->rects[li - c->w.s->n_points];
+        float a = rnd_tgt(R), b = rnd_tgt(R);
+        target = vadd(vadd(L->pos, vmul(L->left, a)), vmul(L->up, b));
+        rnd_unused(R); rnd_unused(R);
+    }
+    return target;
+}
+
+/* Probability::TimeAngular (SamplingTechniques.h:170-250) with eta = 1 and c = 1 / time_of_flight(1)
+ * = 1, so ec = 1: the angle theta between the new direction and the direction to the target such
+ * that a free flight of r followed by the straight line to the target arrives at time t. */
+typedef struct { float t_a, t_b, r, modl; } time_angular;
+
+static time_angular ta_make(float t_a, float t_b, float r, float modl)
+{
+    time_angular s = {t_a, t_b, r, modl};
+    const float ec = 1.0f;
+    if (r > modl) s.t_a = fmaxf(s.t_a, (2.0f * r - modl) * ec);
+    else s.t_a = fmaxf(s.t_a, modl * ec);
+    s.t_b = fminf(s.t_b, (2.0f * r + modl) * ec);
+    return s;
+}
+static float ta_theta(const time_angular* s, float t)
+{
+    const double ec = 1.0;
+    float cos_t = (float)(((double)(s->modl * s->modl) + 2. * (double)t * (double)s->r / ec -
+                           (double)(t * t) / (ec * ec)) / (2. * (double)s->r * (double)s->modl));
+    cos_t = fmaxf(-1.0f, fminf(cos_t, 1.0f));
+    return acosf(cos_t);
+}
+static float ta_cdf_inv(const time_angular* s, float e) { return ta_theta(s, e * (s->t_b - s->t_a) + s->t_a); }
+static float ta_pdf(const time_angular* s, float theta)
+{
+    const float ec = 1.0f;
+    double den = sqrt((double)(s->r * s->r + s->modl * s->modl) - 2.0 * (double)s->r * (double)s->modl * (double)cosf(theta));
+    float dtdtheta = (float)((double)(ec * s->modl * s->r * sinf(theta)) / den);
+    return dtdtheta / (s->t_b - s->t_a);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -584,7 +638,7 @@ static v3 hit_normal(const world_t* w, int tri, v3 p)
 
 typedef struct { float rgb[3]; float time; int bounce; } rsample;
 
-static void generate_path(ctx_t* c, rngs* R, v3 O, v3 D, int max_size, vertex_t* verts, int* nv,
+static void generate_path(ctx_t* c, rngs* R, v3 O, v3 D, v3 target, int max_size, vertex_t* verts, int* nv,
                           orc_stats* st)
 {
     const orc_scene* s = c->w.s;
@@ -600,25 +654,52 @@ static void generate_path(ctx_t* c, rngs* R, v3 O, v3 D, int max_size, vertex_t*
     for (;;) {
         v3 ut = V(0, 0, 0), us = V(0, 0, 0);
         float pd = 1.0f, sampled = INFINITY;
+        const float lam = c->time_sampling ? c->ts_lambda : c->lambda;
         if (c->has_medium) {
             ut = c->st; us = c->ss;
-            /* sample_next_event_distance (241-265), MeanFreePath */
+            /* sample_next_event_distance (241-265): MeanFreePath, or Time in time-sampling mode;
+             * log and exp of a float are the float overloads */
             float eps = rnd_path(R);
-            sampled = (float)(-log((double)eps) / (double)c->lambda);
-            pd = (float)(exp(-(double)(sampled * c->lambda)) * (double)c->lambda);
+            sampled = -logf(eps) / lam;
+            pd = expf(-(sampled * lam)) * lam;
         }
         v3 fa = V(1, 1, 1);
         float pa = 1.0f;
         if (first) {
             first = 0;
         } else if (media_interaction) {
-            /* sample_outgoing_ray, medium (343-398): xi (unused with phase sampling), phase sample */
-            rnd_unused(R);
-            float e1 = rnd_path(R), e2 = rnd_path(R);
-            float pdf;
-            v3 wo = sample_spherical(e1, e2, &pdf);
-            wo = transform_matrix_to(wo, V(0, 1, 0), D);
+            /* sample_outgoing_ray, medium (343-398): xi, then time-angular sampling towards the
+             * target (p_ts = 1, ShortestTime) or a phase-function sample (p_ts = 0: steady-state
+             * mode, or the time window left after this flight is shorter than one exposure) */
             float ph = (float)(.25 * M_1_PI);
+            int phase = 1;
+            v3 wo = V(0, 0, 0);
+            float pdf = 0.0f;
+            if (c->time_sampling) {
+                v3 l = vsub(target, cur_pos);
+                float mod = vlen(l);
+                float ta = mod * 1.0f + time;   /* time_of_flight(mod) * ior + current delay */
+                float tb = c->total_time;
+                time_angular sa = ta_make(ta, tb, sampled, mod);
+                phase = (tb - (ta + sampled * 1.0f)) < c->exposure;
+                rnd_unused(R);                  /* xi: compared with p_ts in {0, 1} */
+                if (!phase) {
+                    float theta = ta_cdf_inv(&sa, rnd_path(R));
+                    float phi = (float)(2. * M_PI * (double)rnd_path(R));
+                    v3 dir = V((float)(cos((double)phi) * sin((double)theta)), (float)cos((double)theta), (float)(sin((double)phi) * sin((double)theta)));
+                    wo = transform_matrix_to(dir, V(0, 1, 0), l);
+                    /* p = phase * (1 - p_ts) + pdf(theta) / (2 pi) * p_ts, divided by p_ts */
+                    pdf = (float)((double)ph * 0.0 + (double)ta_pdf(&sa, theta) / (2. * M_PI) * 1.0);
+                    if (pdf < 1e-5f) phase = 1;
+                }
+            } else {
+                rnd_unused(R);                  /* xi (unused with phase sampling) */
+            }
+            if (phase) {
+                float e1 = rnd_path(R), e2 = rnd_path(R);
+                wo = sample_spherical(e1, e2, &pdf);
+                wo = transform_matrix_to(wo, V(0, 1, 0), D);
+            }
             fa = vmul(c->ss, ph);      /* phase * sigma_s (PhaseFunction.h:78-84 with unit time pdf) */
             pa = pdf;
             O = cur_pos;
@@ -662,7 +743,7 @@ static void generate_path(ctx_t* c, rngs* R, v3 O, v3 D, int max_size, vertex_t*
         } else if (tri >= 0 && thit < sampled) {
             sampled = thit;
             media_interaction = 0;
-            pd = (float)exp(-(double)(sampled * c->lambda));
+            pd = expf(-(sampled * lam));
         } else {
             media_interaction = 1;
         }
@@ -692,13 +773,18 @@ static void generate_path(ctx_t* c, rngs* R, v3 O, v3 D, int max_size, vertex_t*
         }
         vx->pos = cur_pos; vx->dir = D; vx->f = f; vx->time = time;
         (*nv)++;
-        /* sample_termination (447-481): level cap, then Russian roulette on the mean albedo */
+        /* sample_termination (447-481): level cap, then Russian roulette on the mean albedo, or
+         * in time-sampling mode the film's time length (TotalTime, no draw) */
         pi = 1.0f;
         if (level > max_size) return;
-        float a = favg(albedo);
-        float eps = rnd_path(R);
-        if (eps > a) return;
-        pi = a;
+        if (c->time_sampling) {
+            if (time > c->total_time) return;
+        } else {
+            float a = favg(albedo);
+            float eps = rnd_path(R);
+            if (eps > a) return;
+            pi = a;
+        }
         if (*nv >= MAXV) return;
     }
 }
@@ -787,6 +873,13 @@ int orc_render_ex(const orc_scene* s, const orc_render_desc* d, double* film_sum
         c.lambda = favg(c.st);
     }
     c.total_time = (float)d->time_bins * d->exposure + d->offset;
+    c.time_sampling = d->time_sampling != 0;
+    c.exposure = d->exposure;
+    {
+        /* total_time / time_of_flight(1) / ior with time_of_flight(d) = d and ior = 1 (World.h:776-786) */
+        float total = (float)d->time_bins * d->exposure;
+        c.ts_lambda = 4.0f / (total / 1.0f / 1.0f);
+    }
 
     /* camera */
     const float* cm = d->camera;
@@ -860,11 +953,12 @@ int orc_render_ex(const orc_scene* s, const orc_render_desc* d, double* film_sum
                             uint64_t pid = cs * (uint64_t)d->n_incoming + (uint64_t)si;
                             orc_pcg32_seed(&R.path, d->seed, 2 * pid);
                             orc_pcg32_seed(&R.nee, d->seed, 2 * pid + 1);
+                            if (c.time_sampling) orc_pcg32_seed(&R.tgt, d->seed, (3ULL << 61) | pid);
                         }
                         st.paths++;
-                        unused_light_path_draws(&c, &R);
+                        v3 target = light_path_origin(&c, &R);
                         int nvx = 0;
-                        generate_path(&c, &R, cpos, dir, max_size, verts, &nvx, &st);
+                        generate_path(&c, &R, cpos, dir, target, max_size, verts, &nvx, &st);
                         if (nvx == 0) continue;
                         ns += connect_paths(&c, &R, verts, nvx, max_size, inv_inc, smp + ns, &st);
                     }

@@ -68,6 +68,13 @@ typedef struct {
     /* MultibounceStreakFilm (MultibounceStreakCameraFilm.h:67-171) with this many bounce films;
      * 0 = a single StreakCameraFilm */
     int32_t bounce_layers;
+    /* 1: the integrator's Transient sampling mode (-time-sampling, BidirectionalPathTracing.h:190-198):
+     * free flights with lambda = 4 / (T * exposure) (NextEventSampling::Time), directions at medium
+     * vertices sampled towards a light point drawn per path (AngularSampling::ShortestTime,
+     * SamplingTechniques.h:170-250), and termination by the film's time length instead of Russian
+     * roulette (TerminationSampling::TotalTime). In the per-path scheme the light point comes from a
+     * fourth stream, seq = 3 * 2^61 | path id. */
+    int32_t time_sampling;
 } orc_render_desc;
 
 typedef struct {

@@ -42,7 +42,7 @@ class _Render(ctypes.Structure):
         ("seed", ctypes.c_uint64), ("rng_mode", ctypes.c_int32),
         ("row_begin", ctypes.c_int32), ("row_end", ctypes.c_int32),
         ("tnear", ctypes.c_float), ("shadow_tnear", ctypes.c_float), ("shadow_tfar_eps", ctypes.c_float),
-        ("bounce_layers", ctypes.c_int32),
+        ("bounce_layers", ctypes.c_int32), ("time_sampling", ctypes.c_int32),
     ]
 
 
@@ -171,6 +171,7 @@ def render(scene, rng_mode=RNG_PER_PATH, rows=None, seed=None, eps=None, bounce_
         d.row_begin, d.row_end = r0, r1
         nrows = r1 - r0
     d.bounce_layers = int(bounce_layers)
+    d.time_sampling = int(bool(getattr(r, "time_sampling", False)))
     shape = (nrows, f.width, f.time_bins, 3)
     film = np.zeros(((bounce_layers,) if bounce_layers else ()) + shape, dtype=np.float64)
     weight = np.zeros((nrows, f.width), dtype=np.float64)

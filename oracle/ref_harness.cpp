@@ -89,6 +89,7 @@ struct SceneDesc {
     bool has_medium = false;
     double medium[6] = {0, 0, 0, 0, 0, 0};
     int multibounce = 0;  // > 0: MultibounceStreakFilm with this many bounce films (scanline only)
+    bool time_sampling = false;  // -time-sampling: the integrator's Transient mode (bunnykiller.cpp:1811-1817)
     std::string out = "film";
 };
 
@@ -127,6 +128,8 @@ static SceneDesc parse_scene(const char* path)
             s.has_medium = true; for (double& v : s.medium) ls >> v;
         } else if (key == "multibounce") {
             ls >> s.multibounce;
+        } else if (key == "timesampling") {
+            s.time_sampling = true;
         } else if (key == "out") {
             ls >> s.out;
         } else {
@@ -185,8 +188,9 @@ static int run_render_mb(const SceneDesc& s, World<3, Radiance>& w, PinholeCamer
     bpt.set_max_path_size(s.max_bounces);
     bpt.set_path_tracing_only(true);
     bpt.set_scattering_components(Scattering::ALL);
-    bpt.set_mode(BidirectionalPathTracing<3, Radiance, RadianceAttenuation>::SamplingMode::SteadyState);
-    TransientRenderer<3, Radiance, RadianceAttenuation> engine(f_log, false);
+    bpt.set_mode(s.time_sampling ? BidirectionalPathTracing<3, Radiance, RadianceAttenuation>::SamplingMode::Transient
+                                 : BidirectionalPathTracing<3, Radiance, RadianceAttenuation>::SamplingMode::SteadyState);
+    TransientRenderer<3, Radiance, RadianceAttenuation> engine(f_log, s.time_sampling);
     engine.render(s.out.c_str(), w, &bpt, camera, &film, &sampler);
     if (f_log) fclose(f_log);
     // per bounce film: the unnormalised slice [W][T][3] and the weights [W] of the scanline
@@ -274,10 +278,11 @@ static int run_render(const SceneDesc& s, bool reseed_per_camera_sample)
     bpt.set_max_path_size(s.max_bounces);
     bpt.set_path_tracing_only(true);
     bpt.set_scattering_components(Scattering::ALL);
-    bpt.set_mode(BidirectionalPathTracing<3, Radiance, RadianceAttenuation>::SamplingMode::SteadyState);
+    bpt.set_mode(s.time_sampling ? BidirectionalPathTracing<3, Radiance, RadianceAttenuation>::SamplingMode::Transient
+                                 : BidirectionalPathTracing<3, Radiance, RadianceAttenuation>::SamplingMode::SteadyState);
 
     if (!reseed_per_camera_sample) {
-        TransientRenderer<3, Radiance, RadianceAttenuation> engine(f_log, false);
+        TransientRenderer<3, Radiance, RadianceAttenuation> engine(f_log, s.time_sampling);
         engine.render(s.out.c_str(), w, &bpt, camera, &film, &sampler);
     } else {
         // The loop of TransientRenderer::render (TransientRenderer.h:130-179) with the global RNG
@@ -296,7 +301,8 @@ static int run_render(const SceneDesc& s, bool reseed_per_camera_sample)
             ic[0] = meh[0];
             ic[1] = meh[1];
             Ray<3> r = camera.get_ray(ic);
-            w.Li(r, &bpt, rec, film.get_exposure_time(), 0);
+            if (s.time_sampling) w.Li(r, &bpt, rec, film.get_time_length(), film.get_time_resolution());
+            else w.Li(r, &bpt, rec, film.get_exposure_time(), 0);
             for (auto& smp : rec.samples)
                 if (!smp.radiance.is_finite()) smp.radiance = Radiance(0.);
             film.add_samples(fs, rec);

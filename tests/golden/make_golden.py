@@ -89,8 +89,22 @@ def film_cases():
     rect.name = "box_rect_unwarp"
     smooth = scenes.smooth_scene(level=3, width=16, height=16, time_bins=256, sqrt_spp=2, n_incoming=4,
                                  max_bounces=6, seed=7)
+    # the integrator's Transient sampling mode (-time-sampling)
+    box_ts = scenes.cornell_box(width=16, height=16, time_bins=256, sqrt_spp=2, n_incoming=4, max_bounces=6, seed=4)
+    box_ts.render.time_sampling = True
+    media_ts = scenes.media_box(width=24, height=24, scanline=12, time_bins=512, sqrt_spp=4, n_incoming=4,
+                                max_bounces=16, seed=1)
+    media_ts.render.time_sampling = True
+    rmedia_ts = scenes.media_box(width=16, height=16, scanline=6, time_bins=256, sqrt_spp=4, n_incoming=4,
+                                 max_bounces=12, seed=2)
+    rmedia_ts.point_lights = np.zeros((0, 6), dtype=np.float32)
+    rmedia_ts.rect_lights = np.array([[0.65, 0.99, 0.65, 0.65, 0.0, 0.65, 0.3, 0.3, 20.0]], dtype=np.float32)
+    rmedia_ts.render.time_sampling = True
+    rmedia_ts.name = "media_box_rect"
     return [("box_global", box, "render"), ("media_cs", media, "render_cs"), ("ico3_cs", ico, "render_cs"),
-            ("rect_unwarp_cs", rect, "render_cs"), ("smooth_cs", smooth, "render_cs")]
+            ("rect_unwarp_cs", rect, "render_cs"), ("smooth_cs", smooth, "render_cs"),
+            ("box_ts_global", box_ts, "render"), ("media_ts_cs", media_ts, "render_cs"),
+            ("rect_media_ts_cs", rmedia_ts, "render_cs")]
 
 
 def stats_cases():
@@ -98,7 +112,10 @@ def stats_cases():
                              max_bounces=4, seed=0)
     media = scenes.media_box(width=16, height=16, scanline=8, time_bins=256, exposure=0.02, sqrt_spp=4,
                              n_incoming=64, max_bounces=16, seed=0)
-    return [("box", box, 16), ("media", media, 16)]
+    media_ts = scenes.media_box(width=16, height=16, scanline=8, time_bins=256, exposure=0.02, sqrt_spp=4,
+                                n_incoming=64, max_bounces=16, seed=0)
+    media_ts.render.time_sampling = True
+    return [("box", box, 16), ("media", media, 16), ("media_ts", media_ts, 16)]
 
 
 def isect_cases():
@@ -178,6 +195,17 @@ def main(only=None):
         with open(os.path.join(HERE, "manifest.json"), "w") as fh:
             json.dump(manifest, fh, indent=1, default=float)
         return
+    if only is not None and only.startswith("stats:"):
+        tag = only.split(":", 1)[1]
+        with open(os.path.join(HERE, "manifest.json")) as fh:
+            manifest = json.load(fh)
+        with tempfile.TemporaryDirectory() as tmp:
+            for t, sc, k in stats_cases():
+                if t == tag:
+                    write_stats(manifest, t, sc, k, tmp)
+        with open(os.path.join(HERE, "manifest.json"), "w") as fh:
+            json.dump(manifest, fh, indent=1, default=float)
+        return
     if only == "multibounce":
         with open(os.path.join(HERE, "manifest.json")) as fh:
             manifest = json.load(fh)
@@ -201,16 +229,7 @@ def main(only=None):
             manifest["films"][tag] = {"mode": mode, **describe(s)}
             print(tag, film.shape, float(np.nansum(np.where(np.isfinite(film), film, 0))))
         for tag, s, k in stats_cases():
-            films = []
-            for seed in range(k):
-                s.render.seed = seed
-                films.append(run_film(s, "render", tmp, f"{tag}_{seed}").astype(np.float64))
-            f = np.stack(films)
-            np.savez_compressed(os.path.join(HERE, f"stats_{tag}.npz"), mean=f.mean(0).astype(np.float32),
-                                std=f.std(0, ddof=1).astype(np.float32), k=k)
-            s.render.seed = 0
-            manifest["stats"][tag] = {"mode": "render", "seeds": list(range(k)), **describe(s)}
-            print(tag, "stats", f.shape)
+            write_stats(manifest, tag, s, k, tmp)
         for tag, s, rays in isect_cases():
             path = os.path.join(tmp, f"isect_{tag}")
             with open(path + ".txt", "w") as fh:
@@ -226,6 +245,19 @@ def main(only=None):
     write_mb(manifest)
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, default=float)
+
+
+def write_stats(manifest, tag, s, k, tmp):
+    films = []
+    for seed in range(k):
+        s.render.seed = seed
+        films.append(run_film(s, "render", tmp, f"{tag}_{seed}").astype(np.float64))
+    f = np.stack(films)
+    np.savez_compressed(os.path.join(HERE, f"stats_{tag}.npz"), mean=f.mean(0).astype(np.float32),
+                        std=f.std(0, ddof=1).astype(np.float32), k=k)
+    s.render.seed = 0
+    manifest["stats"][tag] = {"mode": "render", "seeds": list(range(k)), **describe(s)}
+    print(tag, "stats", f.shape)
 
 
 def write_images():

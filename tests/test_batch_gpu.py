@@ -25,7 +25,11 @@ def _jobs():
     rect.rect_lights = np.array([[0.65, 0.99, 0.65, 0.65, 0.0, 0.65, 0.3, 0.3, 20.0]], dtype=np.float32)
     rect.film.camera_unwarp = True
     rect.film.offset = 0.5
-    return [box, media, ico, rect]
+    # a job in the Transient sampling mode beside steady-state ones
+    media_ts = scenes.media_box(width=12, height=12, scanline=3, time_bins=200, sqrt_spp=3, n_incoming=2,
+                                max_bounces=10, seed=21)
+    media_ts.render.time_sampling = True
+    return [box, media, ico, rect, media_ts]
 
 
 def _single(R, s, **kw):

@@ -48,7 +48,7 @@ def _slices(prefix, rows, W, T):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["box", "media_scanline"])
+@pytest.mark.parametrize("kind", ["box", "media_scanline", "media_time_sampling"])
 def test_driver_matches_python_host(tmp_path, kind):
     from jarabotransientrendering_amd import renderer
     if kind == "box":
@@ -59,6 +59,7 @@ def test_driver_matches_python_host(tmp_path, kind):
     else:
         s = scenes.media_box(width=16, height=16, scanline=5, time_bins=256, sqrt_spp=2, n_incoming=2,
                              max_bounces=8, seed=5)
+        s.render.time_sampling = kind == "media_time_sampling"
     (tmp_path / "s.txt").write_text(s.to_ref_text("out"))
     r = _run(["s.txt"], tmp_path)
     assert r.returncode == 0, r.stderr
@@ -107,6 +108,8 @@ def _scene_args(s, tmp_path, name="out"):
              "-new-seed", str(r.seed)]
     if f.scanline >= 0:
         args += ["-film-scanline", str(f.scanline)]
+    if s.render.time_sampling:
+        args += ["-time-sampling"]
     if s.medium is not None:
         args += ["-homogeneous-medium", *[repr(float(x)) for x in s.medium], "-pf-isotropic"]
     for p in s.point_lights:
@@ -122,14 +125,14 @@ def test_bunnykiller_rejects_flags_outside_the_path(tmp_path):
     assert "outside" not in r.stderr
     r = _bk(["-name-mesh", "m.obj", "-phong", "1", "1", "1", "10"], tmp_path)
     assert r.returncode == 1 and "-phong is outside" in r.stderr
-    r = _bk(["-time-sampling"], tmp_path)
+    r = _bk(["-steady-state"], tmp_path)
     assert r.returncode == 1 and "outside" in r.stderr
     r = _bk(["-homogeneous-medium", "0", "0", "0", "1", "1", "1", "-pf-hg", "0.5"], tmp_path)
     assert r.returncode == 1 and "-pf-hg" in r.stderr
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["box", "media_rect"])
+@pytest.mark.parametrize("kind", ["box", "media_rect", "media_rect_time_sampling"])
 def test_bunnykiller_command_line_matches_python_host(tmp_path, kind):
     from jarabotransientrendering_amd import renderer
     from tests.test_image_io import decode_rgbe
@@ -141,6 +144,7 @@ def test_bunnykiller_command_line_matches_python_host(tmp_path, kind):
                              n_incoming=2, max_bounces=6, seed=8)
         s.point_lights = np.zeros((0, 6), np.float32)
         s.rect_lights = np.array([[0.65, 0.99, 0.65, 0.65, 0.0, 0.65, 0.3, 0.3, 20.0]], dtype=np.float32)
+        s.render.time_sampling = kind == "media_rect_time_sampling"
     r = _bk(_scene_args(s, tmp_path), tmp_path)
     assert r.returncode == 0, r.stderr
     img, _, _ = renderer.render_scene(s)

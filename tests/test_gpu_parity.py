@@ -143,8 +143,20 @@ def _cases():
     mrect.film.camera_unwarp = True
     smooth = scenes.smooth_scene(level=3, width=16, height=16, time_bins=256, sqrt_spp=2, n_incoming=4,
                                  max_bounces=6, seed=9)
+    # the integrator's Transient sampling mode (-time-sampling)
+    box_ts = scenes.cornell_box(width=16, height=16, time_bins=256, sqrt_spp=2, n_incoming=4, max_bounces=6, seed=5)
+    box_ts.render.time_sampling = True
+    media_ts = scenes.media_box(width=24, height=24, scanline=12, time_bins=512, sqrt_spp=4, n_incoming=4,
+                                max_bounces=16, seed=6)
+    media_ts.render.time_sampling = True
+    mrect_ts = scenes.media_box(width=16, height=16, scanline=7, time_bins=256, sqrt_spp=3, n_incoming=5,
+                                max_bounces=12, seed=8)
+    mrect_ts.point_lights = np.zeros((0, 6), np.float32)
+    mrect_ts.rect_lights = np.array([[0.65, 0.99, 0.65, 0.65, 0.0, 0.65, 0.3, 0.3, 20.0]], dtype=np.float32)
+    mrect_ts.render.time_sampling = True
     return {"box": box, "media": media, "ico3": ico, "rect_unwarp_offset": rect, "media_rect_lastrow": mrect,
-            "smooth_two_sided": smooth}
+            "smooth_two_sided": smooth, "box_time_sampling": box_ts, "media_time_sampling": media_ts,
+            "media_rect_time_sampling": mrect_ts}
 
 
 @pytest.mark.parametrize("name", list(_cases()))
@@ -156,6 +168,17 @@ def test_render_matches_oracle_same_streams(R, oracle, name):
     assert st["paths"] == ost["paths"]
     assert img.shape == ref.shape and np.isfinite(img).all()
     per_pix = np.abs(img - ref).sum(axis=(2, 3)) / np.maximum(np.abs(ref).sum(axis=(2, 3)), 1e-4)
+    if s.render.time_sampling and s.medium is not None:
+        # The time-angular sampler is ill-conditioned near backward scattering: the density goes
+        # with sin(theta), theta = acos(c) with 1 + c ~ 1e-4, so one ulp in a vertex position or a
+        # free-flight distance moves a path's weight by up to ~1e-3 (building the oracle itself
+        # with fused multiply-adds moves single pixels by 1.3e-3). Same streams; per pixel the
+        # films (bin by bin, so a sample moved across a bin edge counts twice) agree to 1e-2
+        # everywhere and to 1e-4 in most pixels.
+        assert per_pix.max() < 1e-2, np.sort(per_pix.ravel())[-10:]
+        assert (per_pix >= 1e-4).sum() <= max(1, 0.25 * per_pix.size), np.sort(per_pix.ravel())[-10:]
+        assert abs(img.sum() / ref.sum() - 1) < 1e-3
+        return
     # identical random streams; float rounding (GPU libm and fused multiply-adds vs the oracle)
     # or an edge hit can send an occasional path elsewhere
     assert (per_pix >= 1e-4).sum() <= max(1, 0.03 * per_pix.size), np.sort(per_pix.ravel())[-10:]
@@ -164,7 +187,7 @@ def test_render_matches_oracle_same_streams(R, oracle, name):
     assert ost["contributions"] >= 0
 
 
-@pytest.mark.parametrize("tag", ["box", "media"])
+@pytest.mark.parametrize("tag", ["box", "media", "media_ts"])
 def test_render_statistics_match_reference(R, golden_manifest, tag):
     from tests.test_oracle import _scene_from_manifest
     entry = golden_manifest["stats"][tag]

@@ -129,6 +129,11 @@ def test_errors(R):
     bad[5] = np.nan
     with pytest.raises(nat.TrError):
         g.params = bad
+    # guiding does not combine with the Transient sampling mode
+    s = scenes.media_box(width=4, height=4, scanline=1, time_bins=16, sqrt_spp=1, n_incoming=1, max_bounces=3)
+    s.render.time_sampling = True
+    with pytest.raises(nat.TrError, match="TIME_SAMPLING"):
+        R.render_scene(s, guide=g)
 
 
 # ---------------------------------------------------------------------------------------------

@@ -50,7 +50,7 @@ def _scene_from_manifest(entry):
     if name == "cornell_box" or name == "box_rect_unwarp":
         s = scenes.cornell_box(f["width"], f["height"], f["time_bins"], f["exposure"], r["sqrt_spp"],
                                r["n_incoming"], r["max_bounces"], r["seed"])
-    elif name == "media_box":
+    elif name in ("media_box", "media_box_rect"):
         s = scenes.media_box(f["width"], f["height"], f["scanline"], f["time_bins"], f["exposure"], r["sqrt_spp"],
                              r["n_incoming"], r["max_bounces"], r["seed"])
     elif name.startswith("smooth"):
@@ -69,6 +69,7 @@ def _scene_from_manifest(entry):
     s.film.offset = float(f["offset"])
     s.film.scanline = int(f["scanline"])
     s.film.bounce_layers = int(f.get("bounce_layers", 0))
+    s.render.time_sampling = bool(r.get("time_sampling", False))
     if entry["medium"] is not None:
         s.medium = np.asarray(entry["medium"], dtype=np.float32)
     import hashlib
@@ -79,7 +80,8 @@ def _scene_from_manifest(entry):
 MODES = {"render": "RNG_REFERENCE", "render_cs": "RNG_REFERENCE_PER_CS"}
 
 
-@pytest.mark.parametrize("tag", ["box_global", "media_cs", "ico3_cs", "rect_unwarp_cs", "smooth_cs"])
+@pytest.mark.parametrize("tag", ["box_global", "media_cs", "ico3_cs", "rect_unwarp_cs", "smooth_cs",
+                                 "box_ts_global", "media_ts_cs", "rect_media_ts_cs"])
 def test_oracle_reproduces_reference_film(oracle, golden_manifest, tag):
     entry = golden_manifest["films"][tag]
     s = _scene_from_manifest(entry)
@@ -139,7 +141,7 @@ def _zscores(oracle, tag, golden_manifest, k=16):
     return z, float(st["mean"][:, :-1].sum()), totals, kr
 
 
-@pytest.mark.parametrize("tag", ["box", "media"])
+@pytest.mark.parametrize("tag", ["box", "media", "media_ts"])
 def test_per_path_rng_matches_reference_statistics(oracle, golden_manifest, tag):
     z, total_ref, totals, kr = _zscores(oracle, tag, golden_manifest)
     # multiple-testing rule: at most 1% of non-empty bins beyond 3 sigma, none beyond 8 sigma
