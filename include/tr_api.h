@@ -209,7 +209,14 @@ enum {
      * vertices drawn by time-angular sampling towards a light point chosen per path
      * (SamplingTechniques.h:170-250), no Russian roulette (paths end at the film's time length or
      * the bounce cap). A per-job flag: jobs of one batch may differ. Not combined with guiding. */
-    TR_RENDER_TIME_SAMPLING = 16u
+    TR_RENDER_TIME_SAMPLING = 16u,
+    /* traversal stack entries of the 4-wide BVH (launch tuning; results do not change). By default
+     * a BVH of at most 65,536 nodes uses 16-bit node indices (half the LDS of 32-bit ones, so more
+     * wavefronts per compute unit). STACK32: 32-bit entries always. STACK_KEYED: closest-hit
+     * stacks keep each entry's box distance beside its 16-bit index, and entries behind the
+     * current hit are dropped without visiting them. */
+    TR_RENDER_STACK32 = 32u,
+    TR_RENDER_STACK_KEYED = 64u
 };
 
 enum {

@@ -246,15 +246,38 @@ __device__ __forceinline__ int32_t trace(const float4* __restrict__ nodes, const
     return hit;
 }
 
+// Traversal stack entry formats of the 4-wide BVH, kept in LDS, one column per thread:
+//   kStack32   : 32-bit node index;
+//   kStack16   : 16-bit node index (BVHs of at most 65,536 nodes): half the LDS, so more
+//                wavefronts fit on a compute unit;
+//   kStackKeyed: 16-bit node index | the upper 16 bits of the child's entry distance (a lower bound
+//                of it, the float truncated to 7 mantissa bits): a closest-hit query skips a popped
+//                child that starts beyond the current hit without fetching the node.
+enum { kStack32 = 0, kStack16 = 1, kStackKeyed = 2 };
+
 // Same query over the 4-wide BVH (Bvh4NodeHost: 8 float4 per node). Children that the ray enters
 // are visited nearest first; the others are pushed far-to-near. Leaves are tested on the spot.
-template <bool ANY>
+// `stack` is the block's LDS stack, `tid` this thread's column and `stride` the block size.
+template <bool ANY, int SK>
 __device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                          const RayPre& r, float tnear, float& tfar, int32_t* stack, int stride)
+                                          const RayPre& r, float tnear, float& tfar, uint32_t* stack, int tid,
+                                          int stride)
 {
+    static_assert(!(ANY && SK == kStackKeyed), "any-hit queries keep no entry distances");
     int32_t hit = -1;
     int32_t node = 0;
     int sp = 0;
+    auto push = [&](int32_t child, float key) {
+        if constexpr (SK == kStack32) {
+            stack[sp * stride + tid] = uint32_t(child);
+        } else if constexpr (SK == kStack16) {
+            // entries 2k and 2k+1 of a thread share one 32-bit word: no two lanes write one word
+            reinterpret_cast<uint16_t*>(stack)[((sp >> 1) * stride + tid) * 2 + (sp & 1)] = uint16_t(child);
+        } else {
+            stack[sp * stride + tid] = (__float_as_uint(key) & 0xFFFF0000u) | uint32_t(child);
+        }
+        ++sp;
+    };
     for (;;) {
         const float4* n = nodes + 8 * size_t(node);
         float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
@@ -274,7 +297,7 @@ __device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, cons
             float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tnear));
             float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar)) * 1.00000036f;
             bool h = t0 <= t1 && cna[k] >= 0;
-            if (h && cna[k] > 0) {  // leaf: test its triangles now
+            if (h && cna[k] > 0 && !(ANY && hit >= 0)) {  // leaf: test its triangles now
                 for (int i = cha[k]; i < cha[k] + cna[k]; ++i) {
                     const float4* t = tris + 3 * size_t(i);
                     float4 q0 = t[0], q1 = t[1], q2 = t[2];
@@ -282,7 +305,7 @@ __device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, cons
                     int32_t id = __float_as_int(q0.w);
                     if (th > tnear && th <= tfar && (th < tfar || hit < 0 || id < hit)) {
                         hit = id;
-                        if (ANY) return hit;
+                        if (ANY) break;
                         tfar = th;
                     }
                 }
@@ -297,22 +320,35 @@ __device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, cons
         float tk = key[a]; key[a] = key[b]; key[b] = tk;                          \
         int ti = idx[a]; idx[a] = idx[b]; idx[b] = ti;                            \
     }
+        if (ANY && hit >= 0) break;
         TR_CSWAP(0, 1) TR_CSWAP(2, 3) TR_CSWAP(0, 2) TR_CSWAP(1, 3) TR_CSWAP(1, 2)
 #undef TR_CSWAP
         // push the farther hit children (far first), descend into the nearest
 #pragma unroll
         for (int k = 3; k >= 1; --k)
-            if (idx[k] >= 0) {
-                stack[sp * stride] = idx[k];
-                ++sp;
-            }
+            if (idx[k] >= 0) push(idx[k], key[k]);
         if (idx[0] >= 0) {
             node = idx[0];
-        } else {
-            if (sp == 0) break;
-            --sp;
-            node = stack[sp * stride];
+            continue;
         }
+        // pop; a keyed entry whose box starts beyond the hit found since it was pushed is dropped
+        // (the same 1 + 2^-21.5 margin as the box test, so no triangle at t <= tfar is skipped)
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            if constexpr (SK == kStack32) {
+                node = int32_t(stack[sp * stride + tid]);
+            } else if constexpr (SK == kStack16) {
+                node = reinterpret_cast<const uint16_t*>(stack)[((sp >> 1) * stride + tid) * 2 + (sp & 1)];
+            } else {
+                const uint32_t e = stack[sp * stride + tid];
+                if (__uint_as_float(e & 0xFFFF0000u) > tfar * 1.00000036f) continue;
+                node = int32_t(e & 0xFFFFu);
+            }
+            found = true;
+            break;
+        }
+        if (!found) break;
     }
     return hit;
 }

@@ -243,12 +243,35 @@ __global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S0, RenderDev R
 }
 
 // ---------------------------------------------------------------------------------------------
+// Traversal variants: the binary BVH, or the 4-wide BVH with 32-bit, 16-bit or keyed 16-bit stack
+// entries (trd::kStack*). Any-hit queries of a keyed launch use the 16-bit format.
+enum { kTravBinary = 0, kTravWide = 1, kTravWide16 = 2, kTravWideKeyed = 3 };
+
+// LDS words per thread of a traversal block for a stack of STACK entries
+template <int TRAV, int STACK>
+constexpr int stack_words() { return TRAV == kTravWide16 ? (STACK + 1) / 2 : STACK; }
+
+template <bool ANY, int TRAV>
+__device__ __forceinline__ int32_t trace_scene(const SceneDev& S, const RayPre& r, float tnear, float& tfar,
+                                               uint32_t* stack)
+{
+    if constexpr (TRAV == kTravBinary)
+        return trace<ANY>(S.nodes, S.tris, r, tnear, tfar, reinterpret_cast<int32_t*>(stack) + threadIdx.x, kTraceBlock);
+    else if constexpr (TRAV == kTravWide)
+        return trace4<ANY, kStack32>(S.nodes4, S.tris, r, tnear, tfar, stack, threadIdx.x, kTraceBlock);
+    else if constexpr (TRAV == kTravWide16 || ANY)
+        return trace4<ANY, kStack16>(S.nodes4, S.tris, r, tnear, tfar, stack, threadIdx.x, kTraceBlock);
+    else
+        return trace4<ANY, kStackKeyed>(S.nodes4, S.tris, r, tnear, tfar, stack, threadIdx.x, kTraceBlock);
+}
+
+// ---------------------------------------------------------------------------------------------
 // extend: closest hit for each queued ray
-template <bool WIDE, int STACK, bool BATCH>
+template <int TRAV, int STACK, bool BATCH>
 __global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv,
                                                              int cur)
 {
-    __shared__ int32_t stack[STACK * kTraceBlock];
+    __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
     const uint32_t count = Wv.counters[cur];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
@@ -274,8 +297,7 @@ __global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S0, const 
             float4 o = Wv.ray_o[slot], d = Wv.ray_d[slot];
             RayPre r = ray_pre(mk(o), mk(d));
             float tfar = __int_as_float(0x7f800000);
-            int32_t tri = WIDE ? trace4<false>(S.nodes4, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock)
-                               : trace<false>(S.nodes, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock);
+            int32_t tri = trace_scene<false, TRAV>(S, r, kSigmaVisibility, tfar, stack);
             Wv.hit[slot] = make_float2(tri >= 0 ? tfar : -1.0f, __int_as_float(tri));
         }
         }
@@ -779,10 +801,12 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
 
 // ---------------------------------------------------------------------------------------------
 // shadow: occlusion test light -> vertex in (eps, dist - eps]; visible contributions go to the film
-template <bool WIDE, int STACK, bool BATCH>
-__global__ void __launch_bounds__(kTraceBlock) shadow_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv)
+// (7 wavefronts per SIMD: the register budget of 72 VGPRs costs two spilled registers and runs
+// faster than 6 wavefronts without spills)
+template <int TRAV, int STACK, bool BATCH>
+__global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv)
 {
-    __shared__ int32_t stack[STACK * kTraceBlock];
+    __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
     const uint32_t count = Wv.counters[C_SHADOW];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0)
@@ -802,8 +826,7 @@ __global__ void __launch_bounds__(kTraceBlock) shadow_kernel(SceneDev S0, const 
                 float4 o = Wv.sh_o[i], d = Wv.sh_d[i];
                 RayPre r = ray_pre(mk(o), mk(d));
                 float tfar = o.w - kSigmaVisibility;
-                bool vis = (WIDE ? trace4<true>(S.nodes4, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock)
-                                 : trace<true>(S.nodes, S.tris, r, kSigmaVisibility, tfar, stack + threadIdx.x, kTraceBlock)) < 0;
+                bool vis = trace_scene<true, TRAV>(S, r, kSigmaVisibility, tfar, stack) < 0;
                 Wv.sh_vis[i] = vis ? 1 : 0;
                 visible += vis ? 1u : 0u;
             }
@@ -964,19 +987,18 @@ __global__ void __launch_bounds__(kBlock) film_merge_kernel(float4* __restrict__
 }
 
 // batch closest hit for the public query entry point
-template <bool WIDE, int STACK>
+template <int TRAV, int STACK>
 __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, const tr_ray* __restrict__ rays,
                                                                 tr_hit* __restrict__ hits, const uint2* __restrict__ ids,
                                                                 uint32_t n)
 {
-    __shared__ int32_t stack[STACK * kTraceBlock];
+    __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         tr_ray q = rays[i];
         RayPre r = ray_pre(mk(q.org[0], q.org[1], q.org[2]), mk(q.dir[0], q.dir[1], q.dir[2]));
         float tfar = q.tfar;
-        int32_t tri = WIDE ? trace4<false>(S.nodes4, S.tris, r, q.tnear, tfar, stack + threadIdx.x, kTraceBlock)
-                           : trace<false>(S.nodes, S.tris, r, q.tnear, tfar, stack + threadIdx.x, kTraceBlock);
+        int32_t tri = trace_scene<false, TRAV>(S, r, q.tnear, tfar, stack);
         tr_hit h;
         if (tri >= 0) {
             uint2 gp = ids[tri];
@@ -997,31 +1019,44 @@ __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, cons
 // The traversal stack holds at most one entry per BVH level; the smallest instantiation that
 // covers the scene's depth keeps LDS per block (and so occupancy) in check.
 // `bound` = entries the traversal stack can hold at most (binary: depth; 4-wide: 3 per level).
-#define TR_STACK_DISPATCH_W(W, bound, KERNEL, GRID, BLOCK, STREAM, ...)                   \
+#define TR_STACK_DISPATCH3(T, bound, B, KERNEL, GRID, BLOCK, STREAM, ...)                \
     do {                                                                                   \
-        if ((bound) < 16) KERNEL<W, 16><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);          \
-        else if ((bound) < 24) KERNEL<W, 24><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);     \
-        else if ((bound) < 32) KERNEL<W, 32><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);     \
-        else KERNEL<W, kBvhStack><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);                \
+        if ((bound) < 16) KERNEL<T, 16, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);       \
+        else if ((bound) < 24) KERNEL<T, 24, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);  \
+        else if ((bound) < 32) KERNEL<T, 32, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);  \
+        else KERNEL<T, kBvhStack, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);             \
     } while (0)
-#define TR_STACK_DISPATCH3(W, bound, B, KERNEL, GRID, BLOCK, STREAM, ...)                \
+#define TR_STACK_DISPATCH3B(T, bound, batch, KERNEL, GRID, BLOCK, STREAM, ...)                             \
+    do {                                                                                                   \
+        if (batch) TR_STACK_DISPATCH3(T, bound, true, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);           \
+        else TR_STACK_DISPATCH3(T, bound, false, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);                \
+    } while (0)
+// extend / shadow launches: `trav` = kTrav* chosen on the host (trav_for)
+#define TR_TRACE_DISPATCH(trav, bound, batch, KERNEL, GRID, BLOCK, STREAM, ...)                             \
+    do {                                                                                                    \
+        switch (trav) {                                                                                     \
+        case kTravBinary: TR_STACK_DISPATCH3B(kTravBinary, bound, batch, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break; \
+        case kTravWide: TR_STACK_DISPATCH3B(kTravWide, bound, batch, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break;     \
+        case kTravWide16: TR_STACK_DISPATCH3B(kTravWide16, bound, batch, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break; \
+        default: TR_STACK_DISPATCH3B(kTravWideKeyed, bound, batch, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break;       \
+        }                                                                                                   \
+    } while (0)
+#define TR_STACK_DISPATCH_T(T, bound, KERNEL, GRID, BLOCK, STREAM, ...)                   \
     do {                                                                                   \
-        if ((bound) < 16) KERNEL<W, 16, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);       \
-        else if ((bound) < 24) KERNEL<W, 24, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);  \
-        else if ((bound) < 32) KERNEL<W, 32, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);  \
-        else KERNEL<W, kBvhStack, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);             \
+        if ((bound) < 16) KERNEL<T, 16><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);          \
+        else if ((bound) < 24) KERNEL<T, 24><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);     \
+        else if ((bound) < 32) KERNEL<T, 32><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);     \
+        else KERNEL<T, kBvhStack><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);                \
     } while (0)
-#define TR_TRACE_DISPATCH(wide, bound, batch, KERNEL, GRID, BLOCK, STREAM, ...)                    \
-    do {                                                                                           \
-        if (wide && batch) TR_STACK_DISPATCH3(true, bound, true, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);    \
-        else if (wide) TR_STACK_DISPATCH3(true, bound, false, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);       \
-        else if (batch) TR_STACK_DISPATCH3(false, bound, true, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);      \
-        else TR_STACK_DISPATCH3(false, bound, false, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);                \
-    } while (0)
-#define TR_STACK_DISPATCH(wide, s, KERNEL, GRID, BLOCK, STREAM, ...)                       \
-    do {                                                                                   \
-        if (wide) TR_STACK_DISPATCH_W(true, (s)->stack4, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);  \
-        else TR_STACK_DISPATCH_W(false, (s)->depth, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__);       \
+// public closest-hit queries: the scene's own traversal choice
+#define TR_STACK_DISPATCH(trav, bound, KERNEL, GRID, BLOCK, STREAM, ...)                                  \
+    do {                                                                                                  \
+        switch (trav) {                                                                                   \
+        case kTravBinary: TR_STACK_DISPATCH_T(kTravBinary, bound, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break; \
+        case kTravWide: TR_STACK_DISPATCH_T(kTravWide, bound, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break;     \
+        case kTravWide16: TR_STACK_DISPATCH_T(kTravWide16, bound, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break; \
+        default: TR_STACK_DISPATCH_T(kTravWideKeyed, bound, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break;       \
+        }                                                                                                 \
     } while (0)
 
 }  // namespace
@@ -1086,7 +1121,20 @@ struct LaunchCfg {
     bool graph = false;             // replay each wavefront's bounce loop from a captured hipGraph
     bool prof = false;              // per-kernel HIP events
     int mode = kShadeRef;
+    // 4-wide traversal stack entries: 0 = 16-bit when the BVH has at most 65,536 nodes (else
+    // 32-bit); 1 = always 32-bit; 2 = keyed 16-bit for closest hits (16-bit for any hits)
+    int stack_fmt = 0;
 };
+
+// launch flags -> stack format
+static int stack_fmt_of(uint32_t flags)
+{
+    return (flags & TR_RENDER_STACK32) ? 1 : (flags & TR_RENDER_STACK_KEYED) ? 2 : 0;
+}
+static uint32_t stack_flags_of(int fmt)
+{
+    return fmt == 1 ? TR_RENDER_STACK32 : fmt == 2 ? TR_RENDER_STACK_KEYED : 0u;
+}
 
 struct WaveBuffers {
     uint32_t cap = 0;
@@ -1540,7 +1588,9 @@ int tr_intersect_batch_device(tr_scene s, const tr_ray* d_rays, uint32_t n, tr_h
     if (n == 0) return TR_OK;
     if (!d_rays || !d_hits) return fail(TR_ERR_INVALID, "tr_intersect_batch_device: null buffer");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
-    TR_STACK_DISPATCH(s->wide, s, intersect_kernel, trace_grid(n), kTraceBlock, st, s->view(), d_rays, d_hits, s->d_ids, n);
+    const int trav = !s->wide ? kTravBinary : s->n_nodes4 <= 65536u ? kTravWide16 : kTravWide;
+    TR_STACK_DISPATCH(trav, s->wide ? s->stack4 : s->depth, intersect_kernel, trace_grid(n), kTraceBlock, st, s->view(),
+                      d_rays, d_hits, s->d_ids, n);
     HIP_TRY(hipGetLastError());
     return TR_OK;
 }
@@ -1880,6 +1930,7 @@ struct Launch {
     bool batch;
     int bounces;
     int stack_bound;
+    int trav_ext, trav_sh;  // kTrav* of the closest-hit and any-hit launches
     uint32_t pix_slots;
     size_t film_lds;
     // profiling
@@ -1946,7 +1997,7 @@ struct Launch {
         int cur = 0;
         for (int b = 0; b < bounces; ++b) {
             if ((rc = begin(TR_K_EXTEND))) return rc;
-            TR_TRACE_DISPATCH(c.wide, stack_bound, batch, extend_kernel, tg, kTraceBlock, stream, S0, dj, Wv, cur);
+            TR_TRACE_DISPATCH(trav_ext, stack_bound, batch, extend_kernel, tg, kTraceBlock, stream, S0, dj, Wv, cur);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADE))) return rc;
             if (batch) {
@@ -1963,7 +2014,7 @@ struct Launch {
             }
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADOW))) return rc;
-            TR_TRACE_DISPATCH(c.wide, stack_bound, batch, shadow_kernel, tg, kTraceBlock, stream, S0, dj, Wv);
+            TR_TRACE_DISPATCH(trav_sh, stack_bound, batch, shadow_kernel, tg, kTraceBlock, stream, S0, dj, Wv);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_FILM))) return rc;
             if (batch)
@@ -1998,12 +2049,18 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
     if (batch && guide) return fail(TR_ERR_INVALID, "render: guided renders take one job");
     uint64_t total = 0;
     int bounces = 0, stack_bound = 0;
+    uint32_t max_nodes4 = 0;
     for (auto& j : jobs) {
         j.dev.path_begin = total;
         total += j.paths;
         bounces = std::max(bounces, j.max_bounces);
         stack_bound = std::max(stack_bound, cfg.wide ? j.scene->stack4 : j.scene->depth);
+        max_nodes4 = std::max(max_nodes4, j.scene->n_nodes4);
     }
+    // 16-bit stack entries hold node indices below 65,536
+    const bool small = max_nodes4 <= 65536u && cfg.stack_fmt != 1;
+    const int trav_ext = !cfg.wide ? kTravBinary : !small ? kTravWide : cfg.stack_fmt == 2 ? kTravWideKeyed : kTravWide16;
+    const int trav_sh = !cfg.wide ? kTravBinary : !small ? kTravWide : kTravWide16;
     const bool mb = jobs[0].dev.R.layers > 0;
     for (auto& j : jobs)
         if (j.dev.R.layers > 0 && (batch || guide))
@@ -2034,6 +2091,8 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
     L.batch = batch;
     L.bounces = bounces;
     L.stack_bound = stack_bound;
+    L.trav_ext = trav_ext;
+    L.trav_sh = trav_sh;
     if (guide) {
         const HostJob& j = jobs[0];
         if (j.max_bounces < 1) return fail(TR_ERR_INVALID, "tr_render_transient_guided: max_bounces must be >= 1");
@@ -2175,6 +2234,7 @@ int render_impl(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_des
     c.wide = (d->flags & TR_RENDER_BINARY_BVH) == 0;
     c.graph = (d->flags & TR_RENDER_GRAPH) != 0;
     c.prof = (d->flags & TR_RENDER_PROFILE) != 0;
+    if (d->flags & (TR_RENDER_STACK32 | TR_RENDER_STACK_KEYED)) c.stack_fmt = stack_fmt_of(d->flags);
     c.mode = guide ? kShadeGuided : ((d->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef);
     return run_jobs(s->stream, s->wave, jobs, c, guide, s->compute_units, s->ev, &s->ev_made, stats, t_start);
 }
@@ -2241,6 +2301,7 @@ LaunchCfg launch_cfg(const tr_batch_s* b, const tr_launch_desc* L)
         c.graph = (L->flags & TR_RENDER_GRAPH) != 0;
         c.prof = (L->flags & TR_RENDER_PROFILE) != 0;
         c.mode = (L->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef;
+        c.stack_fmt = stack_fmt_of(L->flags);
     }
     return c;
 }
@@ -2382,6 +2443,15 @@ int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
             if ((rc = trial(c, &ms))) return rc;
             if (ms < best_ms) best_ms = ms, bestc = c;
         }
+    // stage 3: traversal stack formats
+    for (int fmt : {1, 2}) {
+        LaunchCfg c = bestc;
+        if (!c.wide || c.stack_fmt == fmt) continue;
+        c.stack_fmt = fmt;
+        double ms;
+        if ((rc = trial(c, &ms))) return rc;
+        if (ms < best_ms) best_ms = ms, bestc = c;
+    }
     // the trials accumulated into the films
     for (auto& e : b->entries)
         if ((rc = tr_film_clear(e.f))) return rc;
@@ -2391,7 +2461,7 @@ int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
     if (best) {
         best->wave_paths = bestc.wave;
         best->flags = (bestc.wide ? 0u : TR_RENDER_BINARY_BVH) | (bestc.graph ? TR_RENDER_GRAPH : 0u) |
-                      (bestc.mode == kShadeOnb ? TR_RENDER_ORTHONORMAL_FRAME : 0u);
+                      (bestc.mode == kShadeOnb ? TR_RENDER_ORTHONORMAL_FRAME : 0u) | stack_flags_of(bestc.stack_fmt);
         best->trace_per_cu = bestc.trace_per_cu;
         best->shade_per_cu = bestc.shade_per_cu;
     }

@@ -416,6 +416,7 @@ class TransientRenderer:
         self.binary_bvh = False   # traverse the binary BVH instead of the 4-wide one
         self.orthonormal_frame = False  # Lambertian sampling frame without the reference's quirk
         self.graph = False              # replay each wavefront from a captured hipGraph
+        self.extra_flags = 0            # launch tuning flags (TR_RENDER_STACK32 / TR_RENDER_STACK_KEYED)
         self.last_stats: Optional[dict] = None
 
     def render(self, name: Optional[str], world: World, integrator: Optional[BidirectionalPathTracing],
@@ -432,7 +433,7 @@ class TransientRenderer:
         integrator.preprocess()
         flags = ((nat.TR_RENDER_PROFILE if self.profile else 0) | (nat.TR_RENDER_BINARY_BVH if self.binary_bvh else 0)
                  | (nat.TR_RENDER_ORTHONORMAL_FRAME if self.orthonormal_frame else 0)
-                 | (nat.TR_RENDER_GRAPH if self.graph else 0) | integrator._flags())
+                 | (nat.TR_RENDER_GRAPH if self.graph else 0) | integrator._flags() | int(self.extra_flags))
         d = nat.RenderDesc(sampler.sqrt_spp, integrator.incoming_samples, integrator.max_path_size, 0, int(seed),
                            int(self.wave_paths), flags)
         st = nat.RenderStats()
