@@ -176,6 +176,39 @@ __device__ __forceinline__ float tri_watertight(const RayPre& r, f3 p0, f3 p1, f
     return T / det;
 }
 
+// The same test returning the unnormalised distance: false when the ray misses, else the hit
+// distance is T / det (the caller divides only for candidate hits).
+__device__ __forceinline__ bool tri_watertight_td(const RayPre& r, f3 p0, f3 p1, f3 p2, float* T_out, float* det_out)
+{
+#pragma clang fp contract(off)
+    f3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
+    float Akx = comp(A, r.kx), Aky = comp(A, r.ky), Akz = comp(A, r.kz);
+    float Bkx = comp(B, r.kx), Bky = comp(B, r.ky), Bkz = comp(B, r.kz);
+    float Ckx = comp(C, r.kx), Cky = comp(C, r.ky), Ckz = comp(C, r.kz);
+    float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
+    float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
+    float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    *T_out = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+    *det_out = det;
+    return true;
+}
+
+__device__ __forceinline__ int32_t sel4(int32_t a0, int32_t a1, int32_t a2, int32_t a3, int k)
+{
+    return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
+}
+
 // Closest (ANY = false) or any (ANY = true) hit in (tnear, tfar]. On a closest hit `tfar` is
 // updated. Equal-distance ties resolve to the lower scene triangle index. `stack` points at this
 // thread's first LDS stack slot; consecutive slots are `stride` ints apart.
@@ -289,6 +322,7 @@ __device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, cons
         int cha[4] = {ch.x, ch.y, ch.z, ch.w}, cna[4] = {cn.x, cn.y, cn.z, cn.w};
         float key[4];
         int idx[4];
+        uint32_t leaves = 0;  // children whose box the ray enters and that are leaves
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             float x0 = (lxa[k] - r.o.x) * r.inv.x, x1 = (hxa[k] - r.o.x) * r.inv.x;
@@ -297,33 +331,72 @@ __device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, cons
             float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tnear));
             float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar)) * 1.00000036f;
             bool h = t0 <= t1 && cna[k] >= 0;
-            if (h && cna[k] > 0 && !(ANY && hit >= 0)) {  // leaf: test its triangles now
-                for (int i = cha[k]; i < cha[k] + cna[k]; ++i) {
-                    const float4* t = tris + 3 * size_t(i);
-                    float4 q0 = t[0], q1 = t[1], q2 = t[2];
-                    float th = tri_watertight(r, mk(q0), mk(q1), mk(q2));
-                    int32_t id = __float_as_int(q0.w);
-                    if (th > tnear && th <= tfar && (th < tfar || hit < 0 || id < hit)) {
-                        hit = id;
-                        if (ANY) break;
-                        tfar = th;
-                    }
-                }
-                h = false;
-            }
+            if (h && cna[k] > 0) leaves |= 1u << k;
+            h = h && cna[k] == 0;
             key[k] = h ? t0 : __int_as_float(0x7f800000);
             idx[k] = h ? cha[k] : -1;
         }
-        // sort (key, idx) ascending: 5-comparator network
+        // the triangles of every entered leaf, one per step: a wavefront runs as many steps as its
+        // busiest lane needs, not the sum over child slots of each slot's largest leaf
+        if (leaves) {
+            int k = __builtin_ctz(leaves);
+            int i = sel4(cha[0], cha[1], cha[2], cha[3], k);
+            int e = i + sel4(cna[0], cna[1], cna[2], cna[3], k);
+            for (;;) {
+                const float4* t = tris + 3 * size_t(i);
+                float4 q0 = t[0], q1 = t[1], q2 = t[2];
+                float T, det;
+                if (tri_watertight_td(r, mk(q0), mk(q1), mk(q2), &T, &det)) {
+                    // divide only when an estimate of the distance is near (tnear, tfar]
+                    float te = T * __builtin_amdgcn_rcpf(det);
+                    if (te >= tnear * 0.99999f && te <= tfar * 1.00001f) {
+                        float th = T / det;
+                        int32_t id = __float_as_int(q0.w);
+                        if (th > tnear && th <= tfar && (th < tfar || hit < 0 || id < hit)) {
+                            hit = id;
+                            tfar = th;
+                        }
+                    }
+                }
+                if (ANY && hit >= 0) break;
+                if (++i == e) {
+                    leaves &= leaves - 1u;
+                    if (!leaves) break;
+                    k = __builtin_ctz(leaves);
+                    i = sel4(cha[0], cha[1], cha[2], cha[3], k);
+                    e = i + sel4(cna[0], cna[1], cna[2], cna[3], k);
+                }
+            }
+            if (ANY && hit >= 0) return hit;
+            if constexpr (!ANY) {
+                // inner children behind a hit found in a sibling leaf are not entered
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (!(key[c] <= tfar * 1.00000036f)) {
+                        key[c] = __int_as_float(0x7f800000);
+                        idx[c] = -1;
+                    }
+            }
+        }
+        if constexpr (!ANY) {
+            // sort (key, idx) ascending: 5-comparator network; nearest child first
 #define TR_CSWAP(a, b)                                                            \
     if (key[b] < key[a]) {                                                        \
         float tk = key[a]; key[a] = key[b]; key[b] = tk;                          \
         int ti = idx[a]; idx[a] = idx[b]; idx[b] = ti;                            \
     }
-        if (ANY && hit >= 0) break;
-        TR_CSWAP(0, 1) TR_CSWAP(2, 3) TR_CSWAP(0, 2) TR_CSWAP(1, 3) TR_CSWAP(1, 2)
+            TR_CSWAP(0, 1) TR_CSWAP(2, 3) TR_CSWAP(0, 2) TR_CSWAP(1, 3) TR_CSWAP(1, 2)
 #undef TR_CSWAP
-        // push the farther hit children (far first), descend into the nearest
+        } else {
+            // any hit: order does not matter; move an entered child to slot 0
+#pragma unroll
+            for (int c = 1; c < 4; ++c)
+                if (idx[0] < 0 && idx[c] >= 0) {
+                    idx[0] = idx[c];
+                    idx[c] = -1;
+                }
+        }
+        // push the other entered children (far first), descend into the first
 #pragma unroll
         for (int k = 3; k >= 1; --k)
             if (idx[k] >= 0) push(idx[k], key[k]);
