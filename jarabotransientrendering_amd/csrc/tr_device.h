@@ -111,7 +111,8 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 // ---- ray/box and watertight ray/triangle -----------------------------------------------------
 struct RayPre {
     f3 o, d, inv;
-    int kx, ky, kz;
+    f3 o_rot;  // origin components in (kx, ky, kz) order
+    int rot;   // = kx: offset of the (kx, ky, kz) run in a rotated triangle vertex (TriRot)
     float Sx, Sy, Sz;
 };
 
@@ -132,59 +133,39 @@ __device__ __forceinline__ RayPre ray_pre(f3 o, f3 d)
     int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
     int kx = kz + 1 == 3 ? 0 : kz + 1;
     int ky = kx + 1 == 3 ? 0 : kx + 1;
-    if (comp(d, kz) < 0.0f) {
-        int t = kx;
-        kx = ky;
-        ky = t;
-    }
-    r.kx = kx;
-    r.ky = ky;
-    r.kz = kz;
+    // The paper swaps kx and ky when d[kz] < 0 to keep the winding; that only negates U, V, W, det
+    // and T (each exactly), so the hit test and T / det are the same without it.
+    r.rot = kx;
+    r.o_rot = mk(comp(o, kx), comp(o, ky), comp(o, kz));
     r.Sz = 1.0f / comp(d, kz);
     r.Sx = comp(d, kx) * r.Sz;
     r.Sy = comp(d, ky) * r.Sz;
     return r;
 }
 
+// Rotated triangle record (TriRot, 16 floats = 64 bytes, leaf order): vertex v at floats
+// [5v, 5v + 5) as (x, y, z, x, y), so that the components (kx, ky, kz) = (kz+1, kz+2, kz) mod 3
+// of any ray are three consecutive floats starting at kx; float 15 = scene triangle index bits.
+struct __attribute__((packed, aligned(4))) Packed3 {
+    float x, y, z;
+};
+
 // Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1) 2013, Sec. 4 (no
-// backface culling; double-precision re-evaluation when an edge function is exactly zero).
-// Returns the distance, or -1 when the ray misses the triangle's interior/edges.
-__device__ __forceinline__ float tri_watertight(const RayPre& r, f3 p0, f3 p1, f3 p2)
+// backface culling; double-precision re-evaluation when an edge function is exactly zero), on a
+// rotated triangle record `t`. Returns false when the ray misses the triangle's interior and
+// edges; otherwise the hit distance is T / det (callers divide only for candidate hits).
+__device__ __forceinline__ bool tri_watertight_td(const RayPre& r, const float* __restrict__ t, float* T_out,
+                                                  float* det_out)
 {
     // No fused multiply-adds here: an edge shared by two triangles must evaluate to exactly
     // opposite values in both, which fma(Cx, By, -Cy*Bx) vs fma(Bx, Cy, -By*Cx) does not give.
 #pragma clang fp contract(off)
-    f3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
-    float Akx = comp(A, r.kx), Aky = comp(A, r.ky), Akz = comp(A, r.kz);
-    float Bkx = comp(B, r.kx), Bky = comp(B, r.ky), Bkz = comp(B, r.kz);
-    float Ckx = comp(C, r.kx), Cky = comp(C, r.ky), Ckz = comp(C, r.kz);
-    float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
-    float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
-    float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
-    float U = Cx * By - Cy * Bx;
-    float V = Ax * Cy - Ay * Cx;
-    float W = Bx * Ay - By * Ax;
-    if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
-        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
-        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
-    }
-    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return -1.0f;
-    float det = U + V + W;
-    if (det == 0.0f) return -1.0f;
-    float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
-    return T / det;
-}
-
-// The same test returning the unnormalised distance: false when the ray misses, else the hit
-// distance is T / det (the caller divides only for candidate hits).
-__device__ __forceinline__ bool tri_watertight_td(const RayPre& r, f3 p0, f3 p1, f3 p2, float* T_out, float* det_out)
-{
-#pragma clang fp contract(off)
-    f3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
-    float Akx = comp(A, r.kx), Aky = comp(A, r.ky), Akz = comp(A, r.kz);
-    float Bkx = comp(B, r.kx), Bky = comp(B, r.ky), Bkz = comp(B, r.kz);
-    float Ckx = comp(C, r.kx), Cky = comp(C, r.ky), Ckz = comp(C, r.kz);
+    const Packed3 a = *reinterpret_cast<const Packed3*>(t + r.rot);
+    const Packed3 b = *reinterpret_cast<const Packed3*>(t + 5 + r.rot);
+    const Packed3 c = *reinterpret_cast<const Packed3*>(t + 10 + r.rot);
+    float Akx = a.x - r.o_rot.x, Aky = a.y - r.o_rot.y, Akz = a.z - r.o_rot.z;
+    float Bkx = b.x - r.o_rot.x, Bky = b.y - r.o_rot.y, Bkz = b.z - r.o_rot.z;
+    float Ckx = c.x - r.o_rot.x, Cky = c.y - r.o_rot.y, Ckz = c.z - r.o_rot.z;
     float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
     float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
     float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
@@ -202,6 +183,24 @@ __device__ __forceinline__ bool tri_watertight_td(const RayPre& r, f3 p0, f3 p1,
     *T_out = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
     *det_out = det;
     return true;
+}
+
+// Tests triangle record `t` (scene triangle `id`) against the current closest hit: updates hit and
+// tfar; equal-distance ties go to the lower scene triangle index.
+__device__ __forceinline__ void tri_hit(const RayPre& r, const float* __restrict__ t, float tnear, float& tfar,
+                                        int32_t& hit)
+{
+    float T, det;
+    if (!tri_watertight_td(r, t, &T, &det)) return;
+    // divide only when an estimate of the distance is near (tnear, tfar]
+    const float te = T * __builtin_amdgcn_rcpf(det);
+    if (!(te >= tnear * 0.99999f && te <= tfar * 1.00001f)) return;
+    const float th = T / det;
+    const int32_t id = __float_as_int(t[15]);
+    if (th > tnear && th <= tfar && (th < tfar || hit < 0 || id < hit)) {
+        hit = id;
+        tfar = th;
+    }
 }
 
 __device__ __forceinline__ int32_t sel4(int32_t a0, int32_t a1, int32_t a2, int32_t a3, int k)
@@ -247,15 +246,8 @@ __device__ __forceinline__ int32_t trace(const float4* __restrict__ nodes, const
             if (h && cnt > 0) {
                 int first = side == 0 ? dd.x : dd.y;
                 for (int i = first; i < first + cnt; ++i) {
-                    const float4* t = tris + 3 * size_t(i);
-                    float4 q0 = t[0], q1 = t[1], q2 = t[2];
-                    float th = tri_watertight(r, mk(q0), mk(q1), mk(q2));
-                    int32_t id = __float_as_int(q0.w);
-                    if (th > tnear && th <= tfar && (th < tfar || hit < 0 || id < hit)) {
-                        hit = id;
-                        if (ANY) return hit;
-                        tfar = th;
-                    }
+                    tri_hit(r, reinterpret_cast<const float*>(tris + 4 * size_t(i)), tnear, tfar, hit);
+                    if (ANY && hit >= 0) return hit;
                 }
                 if (side == 0) hl = false;
                 else hr = false;
@@ -291,15 +283,36 @@ enum { kStack32 = 0, kStack16 = 1, kStackKeyed = 2 };
 // Same query over the 4-wide BVH (Bvh4NodeHost: 8 float4 per node). Children that the ray enters
 // are visited nearest first; the others are pushed far-to-near. Leaves are tested on the spot.
 // `stack` is the block's LDS stack, `tid` this thread's column and `stride` the block size.
+// Per-lane state of a 4-wide BVH query between node steps.
+struct Trav4 {
+    int32_t node;  // node to visit next
+    int32_t sp;    // stack entries
+    int32_t hit;   // closest (or any) hit so far, -1 if none
+    float tfar;    // current far bound (the hit distance once there is a hit)
+};
+
+__device__ __forceinline__ void trav4_begin(Trav4& st, float tfar)
+{
+    st.node = 0;
+    st.sp = 0;
+    st.hit = -1;
+    st.tfar = tfar;
+}
+
+// One node of a closest (ANY = false) or any (ANY = true) hit query over the 4-wide BVH
+// (Bvh4NodeHost: 8 float4 per node): the children the ray enters are pushed far-to-near (any hit:
+// in slot order) and the nearest is visited next; entered leaves are tested here. Returns true
+// when the query is complete (st.hit, st.tfar hold the answer). `stack` is the block's LDS stack,
+// `tid` this thread's column and `stride` the block size.
 template <bool ANY, int SK>
-__device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                          const RayPre& r, float tnear, float& tfar, uint32_t* stack, int tid,
-                                          int stride)
+__device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                            const RayPre& r, float tnear, Trav4& st, uint32_t* stack, int tid,
+                                            int stride)
 {
     static_assert(!(ANY && SK == kStackKeyed), "any-hit queries keep no entry distances");
-    int32_t hit = -1;
-    int32_t node = 0;
-    int sp = 0;
+    int32_t hit = st.hit;
+    float tfar = st.tfar;
+    int sp = st.sp;
     auto push = [&](int32_t child, float key) {
         if constexpr (SK == kStack32) {
             stack[sp * stride + tid] = uint32_t(child);
@@ -311,119 +324,126 @@ __device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, cons
         }
         ++sp;
     };
-    for (;;) {
-        const float4* n = nodes + 8 * size_t(node);
-        float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
-        int4 ch = *reinterpret_cast<const int4*>(n + 6);
-        int4 cn = *reinterpret_cast<const int4*>(n + 7);
-        float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-        float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-        float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-        int cha[4] = {ch.x, ch.y, ch.z, ch.w}, cna[4] = {cn.x, cn.y, cn.z, cn.w};
-        float key[4];
-        int idx[4];
-        uint32_t leaves = 0;  // children whose box the ray enters and that are leaves
+    const float4* n = nodes + 8 * size_t(st.node);
+    float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+    int4 ch = *reinterpret_cast<const int4*>(n + 6);
+    int4 cn = *reinterpret_cast<const int4*>(n + 7);
+    float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+    float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+    float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+    int cha[4] = {ch.x, ch.y, ch.z, ch.w}, cna[4] = {cn.x, cn.y, cn.z, cn.w};
+    float key[4];
+    int idx[4];
+    uint32_t leaves = 0;  // children whose box the ray enters and that are leaves
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float x0 = (lxa[k] - r.o.x) * r.inv.x, x1 = (hxa[k] - r.o.x) * r.inv.x;
-            float y0 = (lya[k] - r.o.y) * r.inv.y, y1 = (hya[k] - r.o.y) * r.inv.y;
-            float z0 = (lza[k] - r.o.z) * r.inv.z, z1 = (hza[k] - r.o.z) * r.inv.z;
-            float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tnear));
-            float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar)) * 1.00000036f;
-            bool h = t0 <= t1 && cna[k] >= 0;
-            if (h && cna[k] > 0) leaves |= 1u << k;
-            h = h && cna[k] == 0;
-            key[k] = h ? t0 : __int_as_float(0x7f800000);
-            idx[k] = h ? cha[k] : -1;
+    for (int k = 0; k < 4; ++k) {
+        float x0 = (lxa[k] - r.o.x) * r.inv.x, x1 = (hxa[k] - r.o.x) * r.inv.x;
+        float y0 = (lya[k] - r.o.y) * r.inv.y, y1 = (hya[k] - r.o.y) * r.inv.y;
+        float z0 = (lza[k] - r.o.z) * r.inv.z, z1 = (hza[k] - r.o.z) * r.inv.z;
+        float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tnear));
+        float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar)) * 1.00000036f;
+        bool h = t0 <= t1 && cna[k] >= 0;
+        if (h && cna[k] > 0) leaves |= 1u << k;
+        h = h && cna[k] == 0;
+        key[k] = h ? t0 : __int_as_float(0x7f800000);
+        idx[k] = h ? cha[k] : -1;
+    }
+    // the triangles of every entered leaf, one per step: a wavefront runs as many steps as its
+    // busiest lane needs, not the sum over child slots of each slot's largest leaf
+    if (leaves) {
+        int k = __builtin_ctz(leaves);
+        int i = sel4(cha[0], cha[1], cha[2], cha[3], k);
+        int e = i + sel4(cna[0], cna[1], cna[2], cna[3], k);
+        for (;;) {
+            tri_hit(r, reinterpret_cast<const float*>(tris + 4 * size_t(i)), tnear, tfar, hit);
+            if (ANY && hit >= 0) break;
+            if (++i == e) {
+                leaves &= leaves - 1u;
+                if (!leaves) break;
+                k = __builtin_ctz(leaves);
+                i = sel4(cha[0], cha[1], cha[2], cha[3], k);
+                e = i + sel4(cna[0], cna[1], cna[2], cna[3], k);
+            }
         }
-        // the triangles of every entered leaf, one per step: a wavefront runs as many steps as its
-        // busiest lane needs, not the sum over child slots of each slot's largest leaf
-        if (leaves) {
-            int k = __builtin_ctz(leaves);
-            int i = sel4(cha[0], cha[1], cha[2], cha[3], k);
-            int e = i + sel4(cna[0], cna[1], cna[2], cna[3], k);
-            for (;;) {
-                const float4* t = tris + 3 * size_t(i);
-                float4 q0 = t[0], q1 = t[1], q2 = t[2];
-                float T, det;
-                if (tri_watertight_td(r, mk(q0), mk(q1), mk(q2), &T, &det)) {
-                    // divide only when an estimate of the distance is near (tnear, tfar]
-                    float te = T * __builtin_amdgcn_rcpf(det);
-                    if (te >= tnear * 0.99999f && te <= tfar * 1.00001f) {
-                        float th = T / det;
-                        int32_t id = __float_as_int(q0.w);
-                        if (th > tnear && th <= tfar && (th < tfar || hit < 0 || id < hit)) {
-                            hit = id;
-                            tfar = th;
-                        }
-                    }
-                }
-                if (ANY && hit >= 0) break;
-                if (++i == e) {
-                    leaves &= leaves - 1u;
-                    if (!leaves) break;
-                    k = __builtin_ctz(leaves);
-                    i = sel4(cha[0], cha[1], cha[2], cha[3], k);
-                    e = i + sel4(cna[0], cna[1], cna[2], cna[3], k);
-                }
-            }
-            if (ANY && hit >= 0) return hit;
-            if constexpr (!ANY) {
-                // inner children behind a hit found in a sibling leaf are not entered
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    if (!(key[c] <= tfar * 1.00000036f)) {
-                        key[c] = __int_as_float(0x7f800000);
-                        idx[c] = -1;
-                    }
-            }
+        if (ANY && hit >= 0) {
+            st.hit = hit;
+            st.tfar = tfar;
+            return true;
         }
         if constexpr (!ANY) {
-            // sort (key, idx) ascending: 5-comparator network; nearest child first
+            // inner children behind a hit found in a sibling leaf are not entered
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (!(key[c] <= tfar * 1.00000036f)) {
+                    key[c] = __int_as_float(0x7f800000);
+                    idx[c] = -1;
+                }
+        }
+    }
+    if constexpr (!ANY) {
+        // sort (key, idx) ascending: 5-comparator network; nearest child first
 #define TR_CSWAP(a, b)                                                            \
     if (key[b] < key[a]) {                                                        \
         float tk = key[a]; key[a] = key[b]; key[b] = tk;                          \
         int ti = idx[a]; idx[a] = idx[b]; idx[b] = ti;                            \
     }
-            TR_CSWAP(0, 1) TR_CSWAP(2, 3) TR_CSWAP(0, 2) TR_CSWAP(1, 3) TR_CSWAP(1, 2)
+        TR_CSWAP(0, 1) TR_CSWAP(2, 3) TR_CSWAP(0, 2) TR_CSWAP(1, 3) TR_CSWAP(1, 2)
 #undef TR_CSWAP
-        } else {
-            // any hit: order does not matter; move an entered child to slot 0
+    } else {
+        // any hit: order does not matter; move an entered child to slot 0
 #pragma unroll
-            for (int c = 1; c < 4; ++c)
-                if (idx[0] < 0 && idx[c] >= 0) {
-                    idx[0] = idx[c];
-                    idx[c] = -1;
-                }
-        }
-        // push the other entered children (far first), descend into the first
+        for (int c = 1; c < 4; ++c)
+            if (idx[0] < 0 && idx[c] >= 0) {
+                idx[0] = idx[c];
+                idx[c] = -1;
+            }
+    }
+    // push the other entered children (far first), descend into the first
 #pragma unroll
-        for (int k = 3; k >= 1; --k)
-            if (idx[k] >= 0) push(idx[k], key[k]);
-        if (idx[0] >= 0) {
-            node = idx[0];
-            continue;
-        }
+    for (int k = 3; k >= 1; --k)
+        if (idx[k] >= 0) push(idx[k], key[k]);
+    bool done = false;
+    if (idx[0] >= 0) {
+        st.node = idx[0];
+    } else {
         // pop; a keyed entry whose box starts beyond the hit found since it was pushed is dropped
         // (the same 1 + 2^-21.5 margin as the box test, so no triangle at t <= tfar is skipped)
-        bool found = false;
+        done = true;
         while (sp > 0) {
             --sp;
+            int32_t nd;
             if constexpr (SK == kStack32) {
-                node = int32_t(stack[sp * stride + tid]);
+                nd = int32_t(stack[sp * stride + tid]);
             } else if constexpr (SK == kStack16) {
-                node = reinterpret_cast<const uint16_t*>(stack)[((sp >> 1) * stride + tid) * 2 + (sp & 1)];
+                nd = reinterpret_cast<const uint16_t*>(stack)[((sp >> 1) * stride + tid) * 2 + (sp & 1)];
             } else {
                 const uint32_t e = stack[sp * stride + tid];
                 if (__uint_as_float(e & 0xFFFF0000u) > tfar * 1.00000036f) continue;
-                node = int32_t(e & 0xFFFFu);
+                nd = int32_t(e & 0xFFFFu);
             }
-            found = true;
+            st.node = nd;
+            done = false;
             break;
         }
-        if (!found) break;
     }
-    return hit;
+    st.sp = sp;
+    st.hit = hit;
+    st.tfar = tfar;
+    return done;
+}
+
+// A whole query (public closest-hit entry point): node steps until done.
+template <bool ANY, int SK>
+__device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                          const RayPre& r, float tnear, float& tfar, uint32_t* stack, int tid,
+                                          int stride)
+{
+    Trav4 st;
+    trav4_begin(st, tfar);
+    while (!trace4_step<ANY, SK>(nodes, tris, r, tnear, st, stack, tid, stride)) {
+    }
+    tfar = st.tfar;
+    return st.hit;
 }
 
 }  // namespace trd

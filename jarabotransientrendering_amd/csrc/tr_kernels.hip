@@ -76,7 +76,7 @@ enum {
 struct SceneDev {
     const float4* nodes;       // binary BVH (4 float4 per node)
     const float4* nodes4;      // 4-wide BVH (8 float4 per node)
-    const float4* tris;
+    const float4* tris;        // rotated triangle records in leaf order (4 float4 each, see TriRot)
     const float4* tri_normal;  // per scene triangle: geometric normal xyz, w = material bits (int)
     const float4* tri_vn;      // smooth triangles: p0, e1, e2, n0, n1, n2 (null if none)
     const float4* mats;        // Kd xyz, w = mean albedo used by the roulette
@@ -265,6 +265,48 @@ __device__ __forceinline__ int32_t trace_scene(const SceneDev& S, const RayPre& 
         return trace4<ANY, kStackKeyed>(S.nodes4, S.tris, r, tnear, tfar, stack, threadIdx.x, kTraceBlock);
 }
 
+template <int TRAV>
+constexpr int stack_kind() { return TRAV == kTravWide ? kStack32 : TRAV == kTravWide16 ? kStack16 : kStackKeyed; }
+
+// Per-lane ray refill for the persistent 4-wide traversal kernels: a lane whose query is complete
+// takes the next ray at once instead of waiting for the wavefront's slowest ray. The wavefront
+// claims `chunk` queue entries at a time from the fetch counter and hands them to its idle lanes
+// in lane order. Call with every lane of the wavefront; `busy` marks the lanes with a query.
+struct RayRefill {
+    uint32_t pos = 0, end = 0;  // the wavefront's claimed range of queue entries (uniform)
+    bool drained = false;       // the queue has no entries left (uniform)
+
+    // Assigns queue entries to idle lanes; `start(i)` begins the query of entry i on this lane.
+    template <class Start>
+    __device__ __forceinline__ void run(uint32_t count, uint32_t chunk, uint32_t* fetch, bool& busy, Start start)
+    {
+        const uint32_t lane = __lane_id();
+        const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+        uint64_t need = __ballot(!busy);
+        while (need && !drained) {
+            if (pos >= end) {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(fetch, chunk);
+                b = __shfl(b, 0);
+                if (b >= count) {
+                    drained = true;
+                    break;
+                }
+                pos = b;
+                end = min(b + chunk, count);
+            }
+            const uint32_t avail = end - pos;
+            const uint32_t rank = uint32_t(__popcll(need & below));
+            if (!busy && rank < avail) {
+                start(pos + rank);
+                busy = true;
+            }
+            pos += min(uint32_t(__popcll(need)), avail);
+            need = __ballot(!busy);
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------------------------
 // extend: closest hit for each queued ray
 template <int TRAV, int STACK, bool BATCH>
@@ -280,26 +322,55 @@ __global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S0, const 
         Wv.counters[C_FETCH_SHADOW] = 0;
         atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_EXT), (unsigned long long)count);
     }
-    // persistent grid: each wavefront fetches kFetch queue entries at a time until the queue is
-    // drained (one fetch atomic per `chunk` rays)
-    const uint32_t lane = __lane_id();
     const uint32_t chunk = fetch_chunk(count);
-    for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(Wv.counters + C_FETCH_EXT, chunk);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        for (uint32_t j = 0; j < chunk; j += 64) {
-        const uint32_t i = base + j + lane;
-        if (i < count) {
-            uint32_t slot = Wv.q[cur][i];
-            const SceneDev& S = BATCH ? jobs[Wv.job[slot]].S : S0;
-            float4 o = Wv.ray_o[slot], d = Wv.ray_d[slot];
-            RayPre r = ray_pre(mk(o), mk(d));
-            float tfar = __int_as_float(0x7f800000);
-            int32_t tri = trace_scene<false, TRAV>(S, r, kSigmaVisibility, tfar, stack);
-            Wv.hit[slot] = make_float2(tri >= 0 ? tfar : -1.0f, __int_as_float(tri));
+    if constexpr (TRAV != kTravBinary) {
+        // persistent grid with per-lane refill: every lane steps its own query one node at a time
+        RayRefill rf;
+        bool busy = false;
+        uint32_t slot = 0;
+        RayPre r;
+        Trav4 st;
+        const float4 *nodes = S0.nodes4, *tris = S0.tris;
+        for (;;) {
+            rf.run(count, chunk, Wv.counters + C_FETCH_EXT, busy, [&](uint32_t i) {
+                slot = Wv.q[cur][i];
+                if constexpr (BATCH) {
+                    const SceneDev& S = jobs[Wv.job[slot]].S;
+                    nodes = S.nodes4;
+                    tris = S.tris;
+                }
+                const float4 o = Wv.ray_o[slot], d = Wv.ray_d[slot];
+                r = ray_pre(mk(o), mk(d));
+                trav4_begin(st, __int_as_float(0x7f800000));
+            });
+            if (__ballot(busy) == 0ull) break;  // the queue is drained and every query complete
+            if (busy && trace4_step<false, stack_kind<TRAV>()>(nodes, tris, r, kSigmaVisibility, st, stack, threadIdx.x,
+                                                                kTraceBlock)) {
+                Wv.hit[slot] = make_float2(st.hit >= 0 ? st.tfar : -1.0f, __int_as_float(st.hit));
+                busy = false;
+            }
         }
+    } else {
+        // persistent grid: each wavefront fetches `chunk` queue entries at a time until the queue
+        // is drained (one fetch atomic per `chunk` rays)
+        const uint32_t lane = __lane_id();
+        for (;;) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(Wv.counters + C_FETCH_EXT, chunk);
+            base = __shfl(base, 0);
+            if (base >= count) break;
+            for (uint32_t j = 0; j < chunk; j += 64) {
+                const uint32_t i = base + j + lane;
+                if (i < count) {
+                    uint32_t slot = Wv.q[cur][i];
+                    const SceneDev& S = BATCH ? jobs[Wv.job[slot]].S : S0;
+                    float4 o = Wv.ray_o[slot], d = Wv.ray_d[slot];
+                    RayPre r = ray_pre(mk(o), mk(d));
+                    float tfar = __int_as_float(0x7f800000);
+                    int32_t tri = trace_scene<false, TRAV>(S, r, kSigmaVisibility, tfar, stack);
+                    Wv.hit[slot] = make_float2(tri >= 0 ? tfar : -1.0f, __int_as_float(tri));
+                }
+            }
         }
     }
 }
@@ -814,21 +885,53 @@ __global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, con
     const uint32_t lane = __lane_id();
     uint32_t visible = 0;
     const uint32_t chunk = fetch_chunk(count);
-    for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(Wv.counters + C_FETCH_SHADOW, chunk);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        for (uint32_t j = 0; j < chunk; j += 64) {
-            const uint32_t i = base + j + lane;
-            if (i < count) {
-                const SceneDev& S = BATCH ? jobs[Wv.sh_job[i]].S : S0;
-                float4 o = Wv.sh_o[i], d = Wv.sh_d[i];
-                RayPre r = ray_pre(mk(o), mk(d));
-                float tfar = o.w - kSigmaVisibility;
-                bool vis = trace_scene<true, TRAV>(S, r, kSigmaVisibility, tfar, stack) < 0;
-                Wv.sh_vis[i] = vis ? 1 : 0;
+    if constexpr (TRAV != kTravBinary) {
+        // persistent grid with per-lane refill (see extend_kernel)
+        RayRefill rf;
+        bool busy = false;
+        uint32_t rec = 0;
+        RayPre r;
+        Trav4 st;
+        const float4 *nodes = S0.nodes4, *tris = S0.tris;
+        for (;;) {
+            rf.run(count, chunk, Wv.counters + C_FETCH_SHADOW, busy, [&](uint32_t i) {
+                rec = i;
+                if constexpr (BATCH) {
+                    const SceneDev& S = jobs[Wv.sh_job[i]].S;
+                    nodes = S.nodes4;
+                    tris = S.tris;
+                }
+                const float4 o = Wv.sh_o[i], d = Wv.sh_d[i];
+                r = ray_pre(mk(o), mk(d));
+                trav4_begin(st, o.w - kSigmaVisibility);
+            });
+            if (__ballot(busy) == 0ull) break;
+            // any hit: a keyed launch uses the 16-bit format (its LDS holds either)
+            constexpr int SK = stack_kind<TRAV>() == kStackKeyed ? kStack16 : stack_kind<TRAV>();
+            if (busy && trace4_step<true, SK>(nodes, tris, r, kSigmaVisibility, st, stack, threadIdx.x, kTraceBlock)) {
+                const bool vis = st.hit < 0;
+                Wv.sh_vis[rec] = vis ? 1 : 0;
                 visible += vis ? 1u : 0u;
+                busy = false;
+            }
+        }
+    } else {
+        for (;;) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(Wv.counters + C_FETCH_SHADOW, chunk);
+            base = __shfl(base, 0);
+            if (base >= count) break;
+            for (uint32_t j = 0; j < chunk; j += 64) {
+                const uint32_t i = base + j + lane;
+                if (i < count) {
+                    const SceneDev& S = BATCH ? jobs[Wv.sh_job[i]].S : S0;
+                    float4 o = Wv.sh_o[i], d = Wv.sh_d[i];
+                    RayPre r = ray_pre(mk(o), mk(d));
+                    float tfar = o.w - kSigmaVisibility;
+                    bool vis = trace_scene<true, TRAV>(S, r, kSigmaVisibility, tfar, stack) < 0;
+                    Wv.sh_vis[i] = vis ? 1 : 0;
+                    visible += vis ? 1u : 0u;
+                }
             }
         }
     }
@@ -1510,7 +1613,7 @@ int tr_scene_commit(tr_scene s)
     size_t nn = b.nodes.size(), nt = b.tris.size(), nm = std::max<size_t>(1, s->mats.size() / 4),
            nl = std::max<size_t>(1, s->lights.size());
     int rc;
-    if ((rc = dev_alloc(&s->d_nodes, 4 * nn, &s->dev_bytes)) || (rc = dev_alloc(&s->d_nodes4, 8 * b4.nodes.size(), &s->dev_bytes)) || (rc = dev_alloc(&s->d_tris, 3 * std::max<size_t>(1, nt), &s->dev_bytes)) ||
+    if ((rc = dev_alloc(&s->d_nodes, 4 * nn, &s->dev_bytes)) || (rc = dev_alloc(&s->d_nodes4, 8 * b4.nodes.size(), &s->dev_bytes)) || (rc = dev_alloc(&s->d_tris, 4 * std::max<size_t>(1, nt), &s->dev_bytes)) ||
         (rc = dev_alloc(&s->d_tri_normal, normals.size(), &s->dev_bytes)) || (rc = dev_alloc(&s->d_mats, nm, &s->dev_bytes)) ||
         (rc = dev_alloc(&s->d_lights, nl, &s->dev_bytes)) || (rc = dev_alloc(&s->d_ids, ids.size(), &s->dev_bytes))) {
         s->release_device();
@@ -1519,7 +1622,22 @@ int tr_scene_commit(tr_scene s)
     HIP_TRY(hipMemcpyAsync(s->d_nodes, b.nodes.data(), nn * sizeof(BvhNodeHost), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_nodes4, b4.nodes.data(), b4.nodes.size() * sizeof(Bvh4NodeHost), hipMemcpyHostToDevice,
                            s->stream));
-    if (nt) HIP_TRY(hipMemcpyAsync(s->d_tris, b.tris.data(), nt * sizeof(TriHost), hipMemcpyHostToDevice, s->stream));
+    // rotated triangle records (TriRot): per vertex x y z x y, then the scene triangle index
+    std::vector<float> trot(16 * std::max<size_t>(1, nt), 0.f);
+    for (size_t i = 0; i < nt; ++i) {
+        const TriHost& t = b.tris[i];
+        const float* P[3] = {t.p0, t.p1, t.p2};
+        float* o = &trot[16 * i];
+        for (int v = 0; v < 3; ++v) {
+            o[5 * v + 0] = P[v][0];
+            o[5 * v + 1] = P[v][1];
+            o[5 * v + 2] = P[v][2];
+            o[5 * v + 3] = P[v][0];
+            o[5 * v + 4] = P[v][1];
+        }
+        o[15] = t.p0[3];  // scene triangle index bits
+    }
+    if (nt) HIP_TRY(hipMemcpyAsync(s->d_tris, trot.data(), nt * 16 * sizeof(float), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_tri_normal, normals.data(), normals.size() * sizeof(float4), hipMemcpyHostToDevice, s->stream));
     if (!s->mats.empty())
         HIP_TRY(hipMemcpyAsync(s->d_mats, s->mats.data(), s->mats.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
