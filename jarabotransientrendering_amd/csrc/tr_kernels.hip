@@ -272,6 +272,8 @@ constexpr int stack_kind() { return TRAV == kTravWide ? kStack32 : TRAV == kTrav
 // takes the next ray at once instead of waiting for the wavefront's slowest ray. The wavefront
 // claims `chunk` queue entries at a time from the fetch counter and hands them to its idle lanes
 // in lane order. Call with every lane of the wavefront; `busy` marks the lanes with a query.
+constexpr int kRefillMin = 32;  // idle lanes that trigger a refill (unless the whole wavefront is idle)
+
 struct RayRefill {
     uint32_t pos = 0, end = 0;  // the wavefront's claimed range of queue entries (uniform)
     bool drained = false;       // the queue has no entries left (uniform)
@@ -283,6 +285,9 @@ struct RayRefill {
         const uint32_t lane = __lane_id();
         const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
         uint64_t need = __ballot(!busy);
+        // refill in bulk: ray set-up runs once for many idle lanes instead of once per finished ray
+        // (half a wavefront measured best on the 82k-triangle scene: 13% faster than refilling at once)
+        if (__popcll(need) < kRefillMin && need != ~0ull) return;
         while (need && !drained) {
             if (pos >= end) {
                 uint32_t b = 0;
