@@ -316,3 +316,71 @@ Bvh4 collapse_bvh4(const BvhBuildResult& b)
     r.stack_bound = 3 * depth;
     return r;
 }
+
+namespace {
+
+// float(origin + q * 2^(e - 127)) with q * 2^(e - 127) exact, as the device evaluates it
+float dequant(float origin, uint32_t q, int be)
+{
+    return origin + float(q) * std::ldexp(1.0f, be - 127);
+}
+
+}  // namespace
+
+std::vector<Bvh4QNodeHost> quantize_bvh4(const Bvh4& b4)
+{
+    std::vector<Bvh4QNodeHost> out(b4.nodes.size());
+    for (size_t ni = 0; ni < b4.nodes.size(); ++ni) {
+        const Bvh4NodeHost& n = b4.nodes[ni];
+        Bvh4QNodeHost& q = out[ni];
+        std::memset(&q, 0, sizeof(q));
+        const float* lo[3] = {n.lox, n.loy, n.loz};
+        const float* hi[3] = {n.hix, n.hiy, n.hiz};
+        bool used[4];
+        for (int k = 0; k < 4; ++k) used[k] = n.count[k] >= 0;
+        int be[3];
+        for (int a = 0; a < 3; ++a) {
+            float o = std::numeric_limits<float>::infinity(), h = -std::numeric_limits<float>::infinity();
+            for (int k = 0; k < 4; ++k)
+                if (used[k]) {
+                    o = std::min(o, lo[a][k]);
+                    h = std::max(h, hi[a][k]);
+                }
+            if (!(o <= h)) o = h = 0.f;  // no used child
+            q.origin[a] = o;
+            // smallest grid step 2^(e-127) whose 255th plane reaches the largest hi plane
+            int e = 1;
+            const double ext = double(h) - double(o);
+            if (ext > 0.0) e = std::max(1, std::min(254, int(std::ceil(std::log2(ext / 255.0))) + 127 - 1));
+            while (e < 254 && dequant(o, 255u, e) < h) ++e;
+            be[a] = e;
+        }
+        q.exps = uint32_t(be[0]) | uint32_t(be[1]) << 8 | uint32_t(be[2]) << 16;
+        // byte offsets: lo_x 0, hi_x 4, lo_y 8, hi_y 12 (word 1), lo_z 16, hi_z 20 (word 2)
+        const int lo_off[3] = {0, 8, 16}, hi_off[3] = {4, 12, 20};
+        for (int k = 0; k < 4; ++k) {
+            if (!used[k]) {
+                for (int a = 0; a < 3; ++a) {
+                    q.q[lo_off[a] + k] = 255;
+                    q.q[hi_off[a] + k] = 0;
+                }
+                q.ref[k] = kQEmpty << 29;
+                continue;
+            }
+            for (int a = 0; a < 3; ++a) {
+                const float o = q.origin[a];
+                const double step = std::ldexp(1.0, be[a] - 127);
+                long ql = long(std::floor((double(lo[a][k]) - double(o)) / step));
+                ql = std::max(0L, std::min(255L, ql));
+                while (ql > 0 && dequant(o, uint32_t(ql), be[a]) > lo[a][k]) --ql;
+                long qh = long(std::ceil((double(hi[a][k]) - double(o)) / step));
+                qh = std::max(0L, std::min(255L, qh));
+                while (qh < 255 && dequant(o, uint32_t(qh), be[a]) < hi[a][k]) ++qh;
+                q.q[lo_off[a] + k] = uint8_t(ql);
+                q.q[hi_off[a] + k] = uint8_t(qh);
+            }
+            q.ref[k] = uint32_t(n.child[k]) | (uint32_t(n.count[k]) << 29);
+        }
+    }
+    return out;
+}

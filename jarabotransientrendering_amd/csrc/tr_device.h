@@ -283,6 +283,8 @@ enum { kStack32 = 0, kStack16 = 1, kStackKeyed = 2 };
 // Same query over the 4-wide BVH (Bvh4NodeHost: 8 float4 per node). Children that the ray enters
 // are visited nearest first; the others are pushed far-to-near. Leaves are tested on the spot.
 // `stack` is the block's LDS stack, `tid` this thread's column and `stride` the block size.
+constexpr uint32_t kQEmptyDev = 7u;  // child reference count of an empty slot (Bvh4QNodeHost)
+
 // Per-lane state of a 4-wide BVH query between node steps.
 struct Trav4 {
     int32_t node;  // node to visit next
@@ -299,8 +301,8 @@ __device__ __forceinline__ void trav4_begin(Trav4& st, float tfar)
     st.tfar = tfar;
 }
 
-// One node of a closest (ANY = false) or any (ANY = true) hit query over the 4-wide BVH
-// (Bvh4NodeHost: 8 float4 per node): the children the ray enters are pushed far-to-near (any hit:
+// One node of a closest (ANY = false) or any (ANY = true) hit query over the quantized 4-wide BVH
+// (Bvh4QNodeHost: 4 x 16 bytes per node): the children the ray enters are pushed far-to-near (any hit:
 // in slot order) and the nearest is visited next; entered leaves are tested here. Returns true
 // when the query is complete (st.hit, st.tfar hold the answer). `stack` is the block's LDS stack,
 // `tid` this thread's column and `stride` the block size.
@@ -324,22 +326,32 @@ __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, co
         }
         ++sp;
     };
-    const float4* n = nodes + 8 * size_t(st.node);
-    float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
-    int4 ch = *reinterpret_cast<const int4*>(n + 6);
-    int4 cn = *reinterpret_cast<const int4*>(n + 7);
-    float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-    float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-    float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-    int cha[4] = {ch.x, ch.y, ch.z, ch.w}, cna[4] = {cn.x, cn.y, cn.z, cn.w};
+    // quantized node (Bvh4QNodeHost): origin, grid exponents, child box bytes, child references
+    const uint4* n = reinterpret_cast<const uint4*>(nodes) + 4 * size_t(st.node);
+    const uint4 w0 = n[0], w1 = n[1], w2 = n[2], w3 = n[3];
+    const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+    const float sx = __uint_as_float((w0.w & 0xFFu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23),
+                sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
+    const uint32_t ref[4] = {w3.x, w3.y, w3.z, w3.w};
+    int cha[4], cna[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        cha[k] = int(ref[k] & 0x1FFFFFFFu);
+        const int c = int(ref[k] >> 29);
+        cna[k] = c == int(kQEmptyDev) ? -1 : c;
+    }
     float key[4];
     int idx[4];
     uint32_t leaves = 0;  // children whose box the ray enters and that are leaves
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        float x0 = (lxa[k] - r.o.x) * r.inv.x, x1 = (hxa[k] - r.o.x) * r.inv.x;
-        float y0 = (lya[k] - r.o.y) * r.inv.y, y1 = (hya[k] - r.o.y) * r.inv.y;
-        float z0 = (lza[k] - r.o.z) * r.inv.z, z1 = (hza[k] - r.o.z) * r.inv.z;
+        // plane = origin + q * step: q * step is exact, so the fused form rounds once, as the host's
+        const float lxk = fmaf(float((w1.x >> (8 * k)) & 0xFFu), sx, ox), hxk = fmaf(float((w1.y >> (8 * k)) & 0xFFu), sx, ox);
+        const float lyk = fmaf(float((w1.z >> (8 * k)) & 0xFFu), sy, oy), hyk = fmaf(float((w1.w >> (8 * k)) & 0xFFu), sy, oy);
+        const float lzk = fmaf(float((w2.x >> (8 * k)) & 0xFFu), sz, oz), hzk = fmaf(float((w2.y >> (8 * k)) & 0xFFu), sz, oz);
+        float x0 = (lxk - r.o.x) * r.inv.x, x1 = (hxk - r.o.x) * r.inv.x;
+        float y0 = (lyk - r.o.y) * r.inv.y, y1 = (hyk - r.o.y) * r.inv.y;
+        float z0 = (lzk - r.o.z) * r.inv.z, z1 = (hzk - r.o.z) * r.inv.z;
         float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tnear));
         float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar)) * 1.00000036f;
         bool h = t0 <= t1 && cna[k] >= 0;

@@ -75,7 +75,7 @@ enum {
 
 struct SceneDev {
     const float4* nodes;       // binary BVH (4 float4 per node)
-    const float4* nodes4;      // 4-wide BVH (8 float4 per node)
+    const float4* nodes4;      // quantized 4-wide BVH (4 x 16 bytes per node, Bvh4QNodeHost)
     const float4* tris;        // rotated triangle records in leaf order (4 float4 each, see TriRot)
     const float4* tri_normal;  // per scene triangle: geometric normal xyz, w = material bits (int)
     const float4* tri_vn;      // smooth triangles: p0, e1, e2, n0, n1, n2 (null if none)
@@ -1589,6 +1589,9 @@ int tr_scene_commit(tr_scene s)
     if (b.depth >= kBvhStack) return fail(TR_ERR_INVALID, "tr_scene_commit: BVH deeper than the traversal stack");
     Bvh4 b4 = collapse_bvh4(b);
     if (b4.stack_bound >= kBvhStack) return fail(TR_ERR_INVALID, "tr_scene_commit: 4-wide BVH deeper than the traversal stack");
+    if (b4.nodes.size() >= (size_t(1) << 29) || b.tris.size() >= (size_t(1) << 29))
+        return fail(TR_ERR_INVALID, "tr_scene_commit: more than 2^29 nodes or triangles");
+    const std::vector<Bvh4QNodeHost> q4 = quantize_bvh4(b4);
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->release_device();
     std::vector<float4> normals(n > 0 ? n : 1);
@@ -1618,15 +1621,14 @@ int tr_scene_commit(tr_scene s)
     size_t nn = b.nodes.size(), nt = b.tris.size(), nm = std::max<size_t>(1, s->mats.size() / 4),
            nl = std::max<size_t>(1, s->lights.size());
     int rc;
-    if ((rc = dev_alloc(&s->d_nodes, 4 * nn, &s->dev_bytes)) || (rc = dev_alloc(&s->d_nodes4, 8 * b4.nodes.size(), &s->dev_bytes)) || (rc = dev_alloc(&s->d_tris, 4 * std::max<size_t>(1, nt), &s->dev_bytes)) ||
+    if ((rc = dev_alloc(&s->d_nodes, 4 * nn, &s->dev_bytes)) || (rc = dev_alloc(&s->d_nodes4, 4 * q4.size(), &s->dev_bytes)) || (rc = dev_alloc(&s->d_tris, 4 * std::max<size_t>(1, nt), &s->dev_bytes)) ||
         (rc = dev_alloc(&s->d_tri_normal, normals.size(), &s->dev_bytes)) || (rc = dev_alloc(&s->d_mats, nm, &s->dev_bytes)) ||
         (rc = dev_alloc(&s->d_lights, nl, &s->dev_bytes)) || (rc = dev_alloc(&s->d_ids, ids.size(), &s->dev_bytes))) {
         s->release_device();
         return rc;
     }
     HIP_TRY(hipMemcpyAsync(s->d_nodes, b.nodes.data(), nn * sizeof(BvhNodeHost), hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->d_nodes4, b4.nodes.data(), b4.nodes.size() * sizeof(Bvh4NodeHost), hipMemcpyHostToDevice,
-                           s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_nodes4, q4.data(), q4.size() * sizeof(Bvh4QNodeHost), hipMemcpyHostToDevice, s->stream));
     // rotated triangle records (TriRot): per vertex x y z x y, then the scene triangle index
     std::vector<float> trot(16 * std::max<size_t>(1, nt), 0.f);
     for (size_t i = 0; i < nt; ++i) {
