@@ -877,10 +877,9 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
 
 // ---------------------------------------------------------------------------------------------
 // shadow: occlusion test light -> vertex in (eps, dist - eps]; visible contributions go to the film
-// (7 wavefronts per SIMD: the register budget of 72 VGPRs costs two spilled registers and runs
-// faster than 6 wavefronts without spills)
+// (8 wavefronts per SIMD: 64 VGPRs, no spills)
 template <int TRAV, int STACK, bool BATCH>
-__global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv)
+__global__ void __launch_bounds__(kTraceBlock, 8) shadow_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv)
 {
     __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
     const uint32_t count = Wv.counters[C_SHADOW];
