@@ -149,8 +149,8 @@ def _mean_and_se(R, scene, seeds, **kw):
     return a.mean(0), a.std(0, ddof=1) / np.sqrt(len(seeds)), a
 
 
-@pytest.mark.parametrize("kind", ["box", "media"])
-def test_guided_render_is_unbiased(R, kind):
+@pytest.mark.parametrize("kind,fraction", [("box", 0.5), ("media", 0.5), ("media", 0.1)])
+def test_guided_render_is_unbiased(R, kind, fraction):
     """Guided (network mixture + BSDF, one-sample MIS) and plain renders estimate the same film.
     The network is deliberately mis-trained (sharp lobes in fixed directions) so that a wrong
     density in the MIS weight would show as a bias."""
@@ -166,7 +166,7 @@ def test_guided_render_is_unbiased(R, kind):
     p[k + 3::5] = 4.0                       # kappa ~ 4 (sharp-ish lobes)
     p[k + 0:k + 40:5] = np.linspace(-1, 1, 8)  # lobe directions spread by the output bias
     p[k + 1:k + 40:5] = 1.0
-    guide = R.PathGuide(bsdf_fraction=0.5, steps_per_wave=0, init_seed=1)
+    guide = R.PathGuide(bsdf_fraction=fraction, steps_per_wave=0, init_seed=1)
     guide.params = p
     seeds = list(range(12))
     mg, sg, _ = _mean_and_se(R, s, seeds, guide=guide, wave_paths=4096)
