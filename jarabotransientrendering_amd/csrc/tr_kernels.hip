@@ -276,7 +276,23 @@ constexpr int kRefillMin = 32;  // idle lanes that trigger a refill (unless the 
 
 struct RayRefill {
     uint32_t pos = 0, end = 0;  // the wavefront's claimed range of queue entries (uniform)
+    uint32_t dyn_base = 0;      // first queue entry handed out through the fetch counter
     bool drained = false;       // the queue has no entries left (uniform)
+
+    // Every wavefront starts on its own first chunk (entries [wave * chunk, (wave + 1) * chunk))
+    // without an atomic; later chunks, in queue order after those, come through the fetch counter.
+    // A launch of cheap rays (a handful of chunks per wavefront) otherwise spends its time on one
+    // contended counter; handing chunks out in queue order keeps the wavefronts on neighbouring
+    // rays (and so on the same part of the BVH in cache).
+    __device__ __forceinline__ RayRefill(uint32_t count, uint32_t chunk)
+    {
+        const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+        const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        const uint64_t first = uint64_t(wave) * chunk, all = uint64_t(waves) * chunk;
+        pos = first < count ? uint32_t(first) : count;
+        end = first + chunk < count ? uint32_t(first + chunk) : count;
+        dyn_base = all < count ? uint32_t(all) : count;
+    }
 
     // Assigns queue entries to idle lanes; `start(i)` begins the query of entry i on this lane.
     template <class Start>
@@ -292,7 +308,7 @@ struct RayRefill {
             if (pos >= end) {
                 uint32_t b = 0;
                 if (lane == 0) b = atomicAdd(fetch, chunk);
-                b = __shfl(b, 0);
+                b = __shfl(b, 0) + dyn_base;
                 if (b >= count) {
                     drained = true;
                     break;
@@ -330,7 +346,7 @@ __global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S0, const 
     const uint32_t chunk = fetch_chunk(count);
     if constexpr (TRAV != kTravBinary) {
         // persistent grid with per-lane refill: every lane steps its own query one node at a time
-        RayRefill rf;
+        RayRefill rf(count, chunk);
         bool busy = false;
         uint32_t slot = 0;
         RayPre r;
@@ -891,7 +907,7 @@ __global__ void __launch_bounds__(kTraceBlock, 8) shadow_kernel(SceneDev S0, con
     const uint32_t chunk = fetch_chunk(count);
     if constexpr (TRAV != kTravBinary) {
         // persistent grid with per-lane refill (see extend_kernel)
-        RayRefill rf;
+        RayRefill rf(count, chunk);
         bool busy = false;
         uint32_t rec = 0;
         RayPre r;
