@@ -137,11 +137,27 @@ def rel_mse(x, r, eps_scale=1e-2, trim=0.0):
     return float(np.mean(e))
 
 
-def guided_comparison(width=128, height=128, ref_spp=65536, pass_sqrt=8, checkpoints=(1, 2, 4, 8, 16, 32),
-                      guide_wave=1 << 20, steps_per_wave=16, bsdf_fraction=0.1, learning_rate=3e-3):
-    """Guided versus plain at equal sample counts, timed; guide settings from a sweep on this workload
-    (BSDF/phase fraction 0.5 / 0.2 / 0.1 / 0.05 gave 1.4x / 1.7x / 2.7x / 3.3x on the trimmed relMSE;
-    every setting stays unbiased, the fraction bounds the guided weight at 1 / fraction)."""
+def bins_sum(x, group=8):
+    """Film [rows][W][T][3] with every `group` consecutive time bins summed."""
+    r, w, t, c = x.shape
+    return x[:, :, : t // group * group].reshape(r, w, t // group, group, c).sum(axis=3)
+
+
+def guided_comparison(width=128, height=128, ref_spp=262144, pass_sqrt=16, checkpoints=(4, 8, 16, 32, 64),
+                      train_passes=64, train_sqrt=4, steps_per_wave=8, bsdf_fraction=0.3, learning_rate=3e-3):
+    """Guided versus plain rendering of the same film, timed. The guide first trains on
+    `train_passes` passes of train_sqrt^2 spp (one wavefront each, `steps_per_wave` Adam steps after
+    each); those passes are timed and their film discarded (an untrained network's samples would stay
+    in the estimate). The network is then frozen and both estimators render passes of pass_sqrt^2 spp.
+
+    Metrics against a long plain render (ref_spp, FP32 film with compensated sums):
+      rel_mse_trim   per (pixel, bin, channel) relMSE without the largest 0.1 % of errors (the target
+                     metric: NEE at vertices near the light has unbounded variance, so the untrimmed
+                     value is set by single samples, for both estimators alike);
+      rel_mse        the same, untrimmed;
+      rel_mse_bins8  trimmed, on the film with 8 consecutive time bins summed (less sensitive to
+                     per-bin sparsity at low sample counts);
+      rel_mse_steady untrimmed, on the time-integrated image."""
     from jarabotransientrendering_amd import renderer
     s = guided_scene(width, height)
     world, cam, sampler, film, integ = renderer.setup_scene(s)
@@ -154,15 +170,23 @@ def guided_comparison(width=128, height=128, ref_spp=65536, pass_sqrt=8, checkpo
     for k in range(n_ref):
         eng.render(None, world, integ, cam, film, sampler, seed=100000 + k)
     ref = film.read().astype(np.float64)
-    ref_steady = ref.sum(axis=2)  # time-integrated image
+    ref_steady, ref_b8 = ref.sum(axis=2), bins_sum(ref)
     ref_sec = time.perf_counter() - t0
     sampler.sqrt_spp = pass_sqrt
     pass_paths = s.n_paths() // (s.render.sqrt_spp ** 2) * pass_sqrt ** 2
 
     def run(guide):
-        film.clean()
-        eng.wave_paths = guide_wave if guide is not None else 0
         t_render, out, done = 0.0, [], 0
+        if guide is not None:
+            sampler.sqrt_spp = train_sqrt
+            eng.wave_paths = pass_paths // pass_sqrt ** 2 * train_sqrt ** 2
+            for k in range(train_passes):
+                st = eng.render(None, world, integ, cam, film, sampler, seed=1000000 + k, guide=guide)
+                t_render += st["render_ms"] * 1e-3
+            guide.set_steps(0)
+            sampler.sqrt_spp = pass_sqrt
+        film.clean()
+        eng.wave_paths = pass_paths if guide is not None else 0
         for cp in checkpoints:
             while done < cp:
                 st = eng.render(None, world, integ, cam, film, sampler, seed=done, guide=guide)
@@ -170,25 +194,21 @@ def guided_comparison(width=128, height=128, ref_spp=65536, pass_sqrt=8, checkpo
                 done += 1
             img = film.read()
             out.append({"passes": cp, "spp": cp * pass_sqrt ** 2, "seconds": t_render,
-                        "rel_mse": rel_mse(img, ref), "rel_mse_trim": rel_mse(img, ref, trim=TRIM),
+                        "rel_mse_trim": rel_mse(img, ref, trim=TRIM), "rel_mse": rel_mse(img, ref),
+                        "rel_mse_bins8": rel_mse(bins_sum(img), ref_b8, trim=TRIM),
                         "rel_mse_steady": rel_mse(img.sum(axis=2), ref_steady)})
         return out
 
     plain = run(None)
     guide = renderer.PathGuide(steps_per_wave=steps_per_wave, bsdf_fraction=bsdf_fraction, learning_rate=learning_rate)
     guided = run(guide)
-    # Next-event estimation at vertices close to the light has unbounded variance (irradiance ~ 1/d^2,
-    # in the reference as here), so the plain relMSE is dominated by single samples; the target and
-    # the times use the relMSE without the largest 0.1 % of per-element errors, and both are reported.
-    key = "rel_mse_trim"
-    target = plain[-1][key]
 
-    def time_to(curve):
+    def time_to(curve, key, target):
         prev = None
         for c in curve:
             if c[key] <= target:
                 if prev is None:
-                    return c["seconds"]
+                    return c["seconds"] * c[key] / target  # reached at the first checkpoint: relMSE ~ 1/time
                 # log-log interpolation between checkpoints
                 a = (np.log(target) - np.log(prev[key])) / (np.log(c[key]) - np.log(prev[key]))
                 return float(np.exp(np.log(prev["seconds"]) + a * (np.log(c["seconds"]) - np.log(prev["seconds"]))))
@@ -196,18 +216,28 @@ def guided_comparison(width=128, height=128, ref_spp=65536, pass_sqrt=8, checkpo
         # not reached: extrapolate with relMSE ~ 1/time from the last checkpoint
         return float(curve[-1]["seconds"] * curve[-1][key] / target)
 
-    t_plain, t_guided = time_to(plain), time_to(guided)
+    speedups = {}
+    for key in ("rel_mse_trim", "rel_mse_bins8", "rel_mse_steady"):
+        target = plain[-1][key]
+        speedups[key] = time_to(plain, key, target) / time_to(guided, key, target)
+    key = "rel_mse_trim"
+    t_plain, t_guided = time_to(plain, key, plain[-1][key]), time_to(guided, key, plain[-1][key])
     return {
         "workload": (f"{s.n_tris}-triangle displaced icosphere in the open unit box filled with a homogeneous "
                      f"medium (sigma_a 0.05, sigma_s 0.5), 0.3x0.3 area light; {width}x{height} px x 1024 bins "
                      f"of 0.01; max 16 events; passes of {pass_sqrt ** 2} spp ({pass_paths} paths)"),
         "reference": {"spp": n_ref * 1024, "seconds": ref_sec, "estimator": "plain, orthonormal frame"},
-        "rel_mse_eps": "1e-2 * mean(reference)^2", "rel_mse_steady": "relMSE of the time-integrated image",
-        "guide_wave_paths": guide_wave, "steps_per_wave": steps_per_wave, "bsdf_fraction": bsdf_fraction,
-        "learning_rate": learning_rate, "target_metric": "rel_mse_trim (largest 0.1 % of per-element errors left out)",
-        "target_rel_mse": target,
+        "rel_mse_eps": "1e-2 * mean(reference)^2",
+        "guide_settings": {"bsdf_fraction": bsdf_fraction, "learning_rate": learning_rate,
+                           "steps_per_wave": steps_per_wave, "train_passes": train_passes,
+                           "train_pass_spp": train_sqrt ** 2},
+        "error_ratio_at_equal_spp": {k: guided[-1][k] / plain[-1][k]
+                                     for k in ("rel_mse_trim", "rel_mse_bins8", "rel_mse_steady")},
+        "target_metric": "rel_mse_trim (largest 0.1 % of per-element errors left out)",
+        "target_rel_mse": plain[-1][key],
         "seconds_to_target": {"plain": t_plain, "guided": t_guided},
         "speedup": t_plain / t_guided,
+        "speedup_by_metric": speedups,
         "plain": plain,
         "guided": guided,
         "guide": guide.info(),

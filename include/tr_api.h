@@ -306,6 +306,9 @@ int tr_guide_destroy(tr_guide guide);
 /* Master FP32 parameters, per layer W[in][out] (in-major) then b[out]; setting them resets Adam. */
 int tr_guide_get_params(tr_guide guide, float* params);
 int tr_guide_set_params(tr_guide guide, const float* params);
+/* Adam steps after each wavefront of later guided renders; 0 freezes the network (renders then
+ * collect no training records). */
+int tr_guide_set_steps(tr_guide guide, uint32_t steps_per_wave);
 /* Network outputs for n raw vertex inputs (12 floats each: position normalised to the scene box,
  * incoming direction wo, normal (zero in a medium), time / film length, medium flag, 0); out holds
  * n x 40 floats (lobe i: direction xyz, concentration and weight logits at 5i..5i+4). */

@@ -159,6 +159,7 @@ SIGNATURES = {
     "tr_guide_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "tr_guide_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "tr_guide_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "tr_guide_set_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "tr_guide_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
     "tr_guide_eval_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                             ctypes.POINTER(ctypes.c_float)]),

@@ -511,7 +511,8 @@ int guide_begin_render(tr_guide g, uint32_t wave, uint32_t max_levels, const flo
                        hipStream_t stream, GuideView* view)
 {
     (void)stream;
-    uint32_t tp = std::min<uint32_t>(g->desc.train_paths, wave);
+    // a frozen network (no Adam steps) collects no training records
+    uint32_t tp = g->desc.steps_per_wave ? std::min<uint32_t>(g->desc.train_paths, wave) : 0u;
     size_t nrec = size_t(tp) * max_levels;
     if (tp > g->tp_cap || nrec > g->rec_cap) {
         g->release_render();
@@ -605,6 +606,13 @@ int tr_guide_create(tr_guide* out, const tr_guide_desc* d)
     std::vector<float> p = initial_params(D.init_seed);
     if ((rc = tr_guide_set_params(g, p.data()))) return bail(rc);
     *out = g;
+    return TR_OK;
+}
+
+int tr_guide_set_steps(tr_guide g, uint32_t steps_per_wave)
+{
+    if (!g) return tri::fail(TR_ERR_INVALID, "tr_guide_set_steps: null guide");
+    g->desc.steps_per_wave = steps_per_wave;
     return TR_OK;
 }
 

@@ -397,6 +397,10 @@ class PathGuide:
         nat.check(nat.lib().tr_guide_train_batch(self.handle, nat.as_ptr(rec), len(rec), ctypes.byref(loss)))
         return float(loss.value)
 
+    def set_steps(self, steps_per_wave: int) -> None:
+        """Adam steps after each wavefront of later renders; 0 freezes the network."""
+        nat.check(nat.lib().tr_guide_set_steps(self.handle, int(steps_per_wave)))
+
     def info(self) -> dict:
         i = nat.GuideInfo()
         nat.check(nat.lib().tr_guide_info_get(self.handle, ctypes.byref(i)))

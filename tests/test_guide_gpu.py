@@ -202,3 +202,10 @@ def test_guided_render_trains_online(R):
     assert np.isfinite(info["last_loss"])
     assert np.isfinite(img).all() and img.sum() > 0
     assert st["kernel_launches"]["guide"] == 2 * st["waves"]
+    # frozen: later renders take no Adam step and keep the weights
+    guide.set_steps(0)
+    before = guide.params
+    img2, st2, _ = R.render_scene(s, guide=guide, wave_paths=1 << 13, seed=5)
+    assert guide.info()["steps"] == info["steps"]
+    np.testing.assert_array_equal(guide.params, before)
+    assert np.isfinite(img2).all() and img2.sum() > 0
