@@ -317,6 +317,12 @@ def bench_batch(args) -> dict:
     avg_ms = kern_ms[dom] / max(kern_n[dom], 1)
     bpl = alg[dom] / max(kern_n[dom], 1)
     achieved = bpl / (avg_ms * 1e-3) / 1e9
+    # the other batch mode (shared wavefronts <-> one job after another) with the same shape
+    other = {**launch, "flags": launch["flags"] ^ 128}  # TR_RENDER_PER_JOB
+    t3 = time.perf_counter()
+    other_paths = b.render(other)["paths"]
+    other_rate = other_paths / (time.perf_counter() - t3)
+    small = small_jobs_comparison(renderer)
     # the same jobs one render call at a time
     eng = renderer.TransientRenderer()
     t2 = time.perf_counter()
@@ -348,9 +354,43 @@ def bench_batch(args) -> dict:
         "kernels_ms": {k: round(v, 3) for k, v in kern_ms.items()},
         "sequential": {"paths_per_s": seq_paths / seq_wall, "seconds": seq_wall,
                        "note": "the same 64 jobs rendered one tr_render_transient call at a time"},
+        "other_batch_mode": {"per_job": bool(other["flags"] & 128), "paths_per_s": other_rate},
+        "small_jobs": small,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
+
+
+def small_jobs_comparison(renderer, n_jobs: int = 256, reps: int = 3) -> dict:
+    """Many small jobs (icosphere level 3-4 in the box, 32x32 px, 128 bins, 16 spp, max 8 bounces):
+    one shared sequence of wavefronts against one job after another and one render call per job."""
+    scs = []
+    for k in range(n_jobs):
+        s = scenes.icosphere_box(level=3 + k % 2, width=32, height=32, time_bins=128, exposure=0.08, sqrt_spp=4,
+                                 n_incoming=1, max_bounces=8, seed=1000 + k)
+        scs.append(s)
+    setups = [renderer.setup_scene(s) for s in scs]
+    b = renderer.Batch()
+    for (w, cam, sampler, film, integ), s in zip(setups, scs):
+        b.add(w, integ, cam, film, sampler, seed=s.render.seed)
+    paths = sum(s.n_paths() for s in scs)
+
+    def rate(fn):
+        fn()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return paths * reps / (time.perf_counter() - t)
+
+    eng = renderer.TransientRenderer()
+    shared = rate(lambda: b.render({"flags": 0}))
+    shared_graph = rate(lambda: b.render({"flags": 8}))
+    per_job = rate(lambda: b.render({"flags": 128}))
+    calls = rate(lambda: [eng.render(None, w, integ, cam, film, sampler, seed=s.render.seed)
+                          for (w, cam, sampler, film, integ), s in zip(setups, scs)])
+    return {"jobs": n_jobs, "paths_per_render": paths,
+            "paths_per_s": {"shared_wavefronts": shared, "shared_wavefronts_graph": shared_graph,
+                            "per_job": per_job, "render_calls": calls}}
 
 
 def copy_probe(nbytes: int, reps: int = 3) -> None:

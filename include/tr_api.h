@@ -216,7 +216,10 @@ enum {
      * stacks keep each entry's box distance beside its 16-bit index, and entries behind the
      * current hit are dropped without visiting them. */
     TR_RENDER_STACK32 = 32u,
-    TR_RENDER_STACK_KEYED = 64u
+    TR_RENDER_STACK_KEYED = 64u,
+    /* batches: render the jobs one after another (each with its own wavefronts, single-job kernels
+     * and the LDS-privatised film) instead of in one shared sequence; tr_autotune tries both */
+    TR_RENDER_PER_JOB = 128u
 };
 
 enum {
@@ -252,7 +255,8 @@ typedef struct tr_batch_s* tr_batch;
 
 typedef struct {
     uint32_t wave_paths;    /* paths per wavefront (0 = library default) */
-    uint32_t flags;         /* TR_RENDER_PROFILE | TR_RENDER_BINARY_BVH | TR_RENDER_ORTHONORMAL_FRAME | TR_RENDER_GRAPH */
+    uint32_t flags;         /* TR_RENDER_PROFILE | TR_RENDER_BINARY_BVH | TR_RENDER_ORTHONORMAL_FRAME | TR_RENDER_GRAPH
+                             * | TR_RENDER_STACK32 | TR_RENDER_STACK_KEYED | TR_RENDER_PER_JOB */
     uint32_t trace_per_cu;  /* persistent traversal blocks per compute unit (0 = default 16) */
     uint32_t shade_per_cu;  /* shading blocks per compute unit (0 = default 2) */
 } tr_launch_desc;

@@ -330,9 +330,10 @@ struct RayRefill {
 };
 
 // ---------------------------------------------------------------------------------------------
-// extend: closest hit for each queued ray
+// extend: closest hit for each queued ray (7 wavefronts per SIMD: the batched variant's per-lane
+// scene pointers fit the 72-VGPR budget without spills)
 template <int TRAV, int STACK, bool BATCH>
-__global__ void __launch_bounds__(kTraceBlock) extend_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv,
+__global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv,
                                                              int cur)
 {
     __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
@@ -1248,6 +1249,9 @@ struct LaunchCfg {
     // 4-wide traversal stack entries: 0 = 16-bit when the BVH has at most 65,536 nodes (else
     // 32-bit); 1 = always 32-bit; 2 = keyed 16-bit for closest hits (16-bit for any hits)
     int stack_fmt = 0;
+    // batches: render the jobs one after another (single-job kernels, LDS-privatised film) instead of
+    // in one shared sequence of wavefronts
+    bool per_job = false;
 };
 
 // launch flags -> stack format
@@ -2448,9 +2452,15 @@ LaunchCfg launch_cfg(const tr_batch_s* b, const tr_launch_desc* L)
         c.prof = (L->flags & TR_RENDER_PROFILE) != 0;
         c.mode = (L->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef;
         c.stack_fmt = stack_fmt_of(L->flags);
+        c.per_job = (L->flags & TR_RENDER_PER_JOB) != 0;
     }
     return c;
 }
+
+// Renders the jobs of a batch: one shared sequence of wavefronts, or (per_job) one job after another
+// on the same stream, with their statistics summed.
+int run_batch(tr_batch_s* b, std::vector<HostJob>& jobs, const LaunchCfg& c, tr_render_stats* stats,
+              std::chrono::steady_clock::time_point t0);
 
 // sqrt_spp_cap > 0 lowers every job's sqrt_spp to at most the cap (autotuner trials)
 int batch_jobs(tr_batch b, std::vector<HostJob>* jobs, int sqrt_spp_cap)
@@ -2522,9 +2532,11 @@ int tr_batch_render(tr_batch b, const tr_launch_desc* launch, tr_render_stats* s
     std::vector<HostJob> jobs;
     int rc = batch_jobs(b, &jobs, 0);
     if (rc) return rc;
-    return run_jobs(b->stream, b->wave, jobs, launch_cfg(b, launch), nullptr, b->compute_units, b->ev, &b->ev_made,
-                    stats, t0);
+    return run_batch(b, jobs, launch_cfg(b, launch), stats, t0);
 }
+
+// a trial replaces the best shape only when it is faster by more than run-to-run noise
+constexpr double kTuneMargin = 0.98;
 
 int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
 {
@@ -2556,30 +2568,34 @@ int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
         for (int rep = 0; rep < 2; ++rep) {
             tr_render_stats st;
             auto t0 = std::chrono::steady_clock::now();
-            int r = run_jobs(b->stream, b->wave, jobs, c, nullptr, b->compute_units, b->ev, &b->ev_made, &st, t0);
+            int r = run_batch(b, jobs, c, &st, t0);
             if (r) return r;
             m = std::min(m, st.render_ms);
         }
         *ms = m;
         return TR_OK;
     };
+    // the current shape is the incumbent; stage 1: wavefront size and graph replay; stage 2: persistent
+    // grid shapes; stage 3: stack formats
     LaunchCfg bestc = base;
-    double best_ms = 1e300;
-    // stage 1: wavefront size and graph replay; stage 2: persistent grid shapes
+    bestc.prof = false;
+    double best_ms;
+    if ((rc = trial(bestc, &best_ms))) return rc;
     std::vector<uint32_t> waves;
     for (uint32_t w : {1u << 18, 1u << 20, 1u << 22, 1u << 24, 1u << 25})
         if (w == (1u << 18) || w / 2 < trial_total) waves.push_back(w);
     for (uint32_t w : waves)
         for (bool g : {false, true}) {
+            if (w == base.wave && g == base.graph) continue;
             LaunchCfg c = base;
             c.wave = w;
             c.graph = g;
             c.prof = false;
             double ms;
             if ((rc = trial(c, &ms))) return rc;
-            if (ms < best_ms) best_ms = ms, bestc = c;
+            if (ms < best_ms * kTuneMargin) best_ms = ms, bestc = c;
         }
-    for (uint32_t tpc : {2u, 4u, 8u, 16u, 32u})
+    for (uint32_t tpc : {8u, 16u, 32u})
         for (uint32_t spc : {1u, 2u, 4u}) {
             if (tpc == bestc.trace_per_cu && spc == bestc.shade_per_cu) continue;
             LaunchCfg c = bestc;
@@ -2587,16 +2603,32 @@ int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
             c.shade_per_cu = spc;
             double ms;
             if ((rc = trial(c, &ms))) return rc;
-            if (ms < best_ms) best_ms = ms, bestc = c;
+            if (ms < best_ms * kTuneMargin) best_ms = ms, bestc = c;
         }
-    // stage 3: traversal stack formats
+    // stage 3: one job after another with the best shape (jobs large enough to fill the GPU on
+    // their own gain the single-job kernels and the LDS-privatised film). Per-job launch overheads
+    // shrink with the job sizes, so this is decided on the full jobs, one run each.
+    if (jobs.size() > 1) {
+        std::vector<HostJob> full;
+        if ((rc = batch_jobs(b, &full, 0))) return rc;
+        double ms[2];
+        for (int pj = 0; pj < 2; ++pj) {
+            LaunchCfg c = bestc;
+            c.per_job = pj != 0;
+            tr_render_stats st;
+            if ((rc = run_batch(b, full, c, &st, std::chrono::steady_clock::now()))) return rc;
+            ms[pj] = st.render_ms;
+        }
+        if (ms[1] < ms[0] * kTuneMargin) bestc.per_job = true;
+    }
+    // stage 4: traversal stack formats
     for (int fmt : {1, 2}) {
         LaunchCfg c = bestc;
         if (!c.wide || c.stack_fmt == fmt) continue;
         c.stack_fmt = fmt;
         double ms;
         if ((rc = trial(c, &ms))) return rc;
-        if (ms < best_ms) best_ms = ms, bestc = c;
+        if (ms < best_ms * kTuneMargin) best_ms = ms, bestc = c;
     }
     // the trials accumulated into the films
     for (auto& e : b->entries)
@@ -2607,7 +2639,8 @@ int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
     if (best) {
         best->wave_paths = bestc.wave;
         best->flags = (bestc.wide ? 0u : TR_RENDER_BINARY_BVH) | (bestc.graph ? TR_RENDER_GRAPH : 0u) |
-                      (bestc.mode == kShadeOnb ? TR_RENDER_ORTHONORMAL_FRAME : 0u) | stack_flags_of(bestc.stack_fmt);
+                      (bestc.mode == kShadeOnb ? TR_RENDER_ORTHONORMAL_FRAME : 0u) | stack_flags_of(bestc.stack_fmt) |
+                      (bestc.per_job ? TR_RENDER_PER_JOB : 0u);
         best->trace_per_cu = bestc.trace_per_cu;
         best->shade_per_cu = bestc.shade_per_cu;
     }
@@ -2616,3 +2649,34 @@ int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
 }
 
 }  // extern "C"
+
+namespace {
+
+int run_batch(tr_batch_s* b, std::vector<HostJob>& jobs, const LaunchCfg& c, tr_render_stats* stats,
+              std::chrono::steady_clock::time_point t0)
+{
+    if (!c.per_job || jobs.size() == 1)
+        return run_jobs(b->stream, b->wave, jobs, c, nullptr, b->compute_units, b->ev, &b->ev_made, stats, t0);
+    tr_render_stats total{};
+    for (auto& j : jobs) {
+        std::vector<HostJob> one(1, j);
+        tr_render_stats st{};
+        int rc = run_jobs(b->stream, b->wave, one, c, nullptr, b->compute_units, b->ev, &b->ev_made, &st, t0);
+        if (rc) return rc;
+        total.paths += st.paths;
+        total.extension_rays += st.extension_rays;
+        total.shadow_rays += st.shadow_rays;
+        total.contributions += st.contributions;
+        total.waves += st.waves;
+        total.bounces_max = std::max(total.bounces_max, st.bounces_max);
+        for (int k = 0; k < TR_K_COUNT; ++k) {
+            total.kernel_ms[k] += st.kernel_ms[k];
+            total.kernel_launches[k] += st.kernel_launches[k];
+        }
+    }
+    total.render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = total;
+    return TR_OK;
+}
+
+}  // namespace

@@ -86,6 +86,18 @@ def test_graph_replay_equals_direct_launches(R):
         np.testing.assert_allclose(x, _single(R, sc)[0], rtol=2e-5, atol=1e-7)
 
 
+def test_per_job_mode_equals_single_renders(R):
+    """TR_RENDER_PER_JOB renders the jobs one after another; the films and totals are unchanged."""
+    from jarabotransientrendering_amd import _native as nat
+    scs = _jobs()
+    imgs, st, _ = _batch(R, scs, launch={"flags": nat.TR_RENDER_PER_JOB})
+    ref = [_single(R, s) for s in scs]
+    assert st["paths"] == sum(s.n_paths() for s in scs)
+    assert st["extension_rays"] == sum(r[1]["extension_rays"] for r in ref)
+    for a, (b, _) in zip(imgs, ref):
+        np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-7)
+
+
 def test_autotune_then_render(R):
     scs = _jobs()
     imgs, st, (best, ms) = _batch(R, scs, autotune=True)
