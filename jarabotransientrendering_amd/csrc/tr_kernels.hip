@@ -15,9 +15,10 @@
 //                                                                            StreakCameraFilm.h:139-192)
 //   film_merge_kernel             pass film -> Neumaier-compensated FP32 film (once per render call)
 //
-// Path state lives in HBM as structure-of-arrays float4/uint4 records indexed by a path slot;
-// queues hold slot indices. Queue counters live in device memory so no host round trip is needed
-// between kernels.
+// Path state lives in HBM as structure-of-arrays float4/uint4 records in two buffers: the rays of
+// bounce k are records [0, count) of buffer k & 1, and shade writes the paths that continue,
+// compacted in queue order, into the other buffer, so every kernel reads and writes its records
+// densely. Queue counters live in device memory so no host round trip is needed between kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -111,19 +112,19 @@ struct RenderDev {
 };
 
 struct WaveDev {
-    float4* ray_o;    // origin xyz, w = sampled free-flight distance of this segment (inf without medium)
-    float4* ray_d;    // direction xyz, w = 0
-    float4* thr;      // throughput xyz (with the pending BSDF/phase weight and roulette), w = time
-    uint4* misc;      // x = path id lo, y = path id hi, z/w = path-stream PCG32 state
-    uint4* extra;     // x/y = light-stream PCG32 state, z = level | film pixel << 8, w = unwarp distance (float bits)
-    float2* hit;      // x = distance, y = scene triangle (int bits) or -1
-    uint32_t* q[2];
+    // path records of the current ([cur]) and the next ([cur ^ 1]) bounce
+    float4* ray_o[2]; // origin xyz, w = sampled free-flight distance of this segment (inf without medium)
+    float4* ray_d[2]; // direction xyz, w = batched renders: job (uint bits); otherwise the path's
+                      // index in its wavefront (uint bits)
+    float4* thr[2];   // throughput xyz (with the pending BSDF/phase weight and roulette), w = time
+    uint4* misc[2];   // x = path id lo, y = path id hi, z/w = path-stream PCG32 state
+    uint4* extra[2];  // x/y = light-stream PCG32 state, z = level | film pixel << 8, w = unwarp distance (float bits)
+    float2* hit;      // per record of the current bounce: x = distance, y = scene triangle (int bits) or -1
     float4* sh_o;     // light point xyz, w = distance
     float4* sh_d;     // direction xyz, w = film element index (uint bits)
     float4* sh_c;     // contribution rgb, w = film pixel (uint bits)
     uint8_t* sh_vis;  // 1 if the shadow ray is unoccluded
     uint32_t* sh_tag; // guided renders: training record of the contribution (or ~0)
-    uint32_t* job;    // batched renders: job of each path slot
     uint32_t* sh_job; // batched renders: job of each shadow record
     struct WaveCtl* ctl;
     uint32_t* counters;
@@ -197,7 +198,6 @@ __global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S0, RenderDev R
         }
         jid = lo;
         pid_local = uint32_t(gidx - jobs[lo].path_begin);
-        Wv.job[slot] = jid;
     } else {
         pid_local = uint32_t(ctl.base) + slot;
     }
@@ -235,12 +235,11 @@ __global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S0, RenderDev R
     if (S.has_medium) {
         sampled = free_flight(pcg_float(pst, pinc), R.time_sampling ? R.ts_lambda : S.lambda);
     }
-    Wv.ray_o[slot] = make_float4(R.cpos.x, R.cpos.y, R.cpos.z, sampled);
-    Wv.ray_d[slot] = make_float4(dir.x, dir.y, dir.z, 0.0f);
-    Wv.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    Wv.misc[slot] = make_uint4(uint32_t(pid), uint32_t(pid >> 32), uint32_t(pst), uint32_t(pst >> 32));
-    Wv.extra[slot] = make_uint4(uint32_t(nst), uint32_t(nst >> 32), pix_local << 8, __float_as_uint(3.402823466e+38f));
-    Wv.q[0][slot] = slot;
+    Wv.ray_o[0][slot] = make_float4(R.cpos.x, R.cpos.y, R.cpos.z, sampled);
+    Wv.ray_d[0][slot] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(BATCH ? jid : slot));
+    Wv.thr[0][slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    Wv.misc[0][slot] = make_uint4(uint32_t(pid), uint32_t(pid >> 32), uint32_t(pst), uint32_t(pst >> 32));
+    Wv.extra[0][slot] = make_uint4(uint32_t(nst), uint32_t(nst >> 32), pix_local << 8, __float_as_uint(3.402823466e+38f));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -350,26 +349,26 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
         // persistent grid with per-lane refill: every lane steps its own query one node at a time
         RayRefill rf(count, chunk);
         bool busy = false;
-        uint32_t slot = 0;
+        uint32_t rec = 0;
         RayPre r;
         Trav4 st;
         const float4 *nodes = S0.nodes4, *tris = S0.tris;
         for (;;) {
             rf.run(count, chunk, Wv.counters + C_FETCH_EXT, busy, [&](uint32_t i) {
-                slot = Wv.q[cur][i];
+                rec = i;
+                const float4 o = Wv.ray_o[cur][i], d = Wv.ray_d[cur][i];
                 if constexpr (BATCH) {
-                    const SceneDev& S = jobs[Wv.job[slot]].S;
+                    const SceneDev& S = jobs[__float_as_uint(d.w)].S;
                     nodes = S.nodes4;
                     tris = S.tris;
                 }
-                const float4 o = Wv.ray_o[slot], d = Wv.ray_d[slot];
                 r = ray_pre(mk(o), mk(d));
                 trav4_begin(st, __int_as_float(0x7f800000));
             });
             if (__ballot(busy) == 0ull) break;  // the queue is drained and every query complete
             if (busy && trace4_step<false, stack_kind<TRAV>()>(nodes, tris, r, kSigmaVisibility, st, stack, threadIdx.x,
                                                                 kTraceBlock)) {
-                Wv.hit[slot] = make_float2(st.hit >= 0 ? st.tfar : -1.0f, __int_as_float(st.hit));
+                Wv.hit[rec] = make_float2(st.hit >= 0 ? st.tfar : -1.0f, __int_as_float(st.hit));
                 busy = false;
             }
         }
@@ -385,13 +384,12 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
             for (uint32_t j = 0; j < chunk; j += 64) {
                 const uint32_t i = base + j + lane;
                 if (i < count) {
-                    uint32_t slot = Wv.q[cur][i];
-                    const SceneDev& S = BATCH ? jobs[Wv.job[slot]].S : S0;
-                    float4 o = Wv.ray_o[slot], d = Wv.ray_d[slot];
+                    float4 o = Wv.ray_o[cur][i], d = Wv.ray_d[cur][i];
+                    const SceneDev& S = BATCH ? jobs[__float_as_uint(d.w)].S : S0;
                     RayPre r = ray_pre(mk(o), mk(d));
                     float tfar = __int_as_float(0x7f800000);
                     int32_t tri = trace_scene<false, TRAV>(S, r, kSigmaVisibility, tfar, stack);
-                    Wv.hit[slot] = make_float2(tri >= 0 ? tfar : -1.0f, __int_as_float(tri));
+                    Wv.hit[i] = make_float2(tri >= 0 ? tfar : -1.0f, __int_as_float(tri));
                 }
             }
         }
@@ -590,8 +588,10 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
         bool live = i < count;
-        uint32_t slot = live ? Wv.q[cur][i] : 0u;
-        const uint32_t jid = (BATCH && live) ? Wv.job[slot] : 0u;
+        const float4 d4 = live ? Wv.ray_d[cur][i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // batched renders: the path's job; otherwise its index in the wavefront
+        const uint32_t slot = __float_as_uint(d4.w);
+        const uint32_t jid = BATCH ? slot : 0u;
         const SceneDev& S = BATCH ? jobs[jid].S : S0;
         const RenderDev& R = BATCH ? jobs[jid].R : R0;
         bool emit_shadow = false, cont = false, want = false;
@@ -607,10 +607,10 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
         uint64_t pst = 0, nst = 0, pinc = 0;
         uint4 misc = {}, extra = {};
         if (live) {
-            float4 o4 = Wv.ray_o[slot], d4 = Wv.ray_d[slot], thr = Wv.thr[slot];
-            misc = Wv.misc[slot];
-            extra = Wv.extra[slot];
-            float2 h = Wv.hit[slot];
+            float4 o4 = Wv.ray_o[cur][i], thr = Wv.thr[cur][i];
+            misc = Wv.misc[cur][i];
+            extra = Wv.extra[cur][i];
+            float2 h = Wv.hit[i];
             int32_t tri = __float_as_int(h.y);
             uint32_t level = extra.z & 0xFFu;
             f3 o = mk(o4);
@@ -867,7 +867,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                 f3 nf = mul(f, fa) * (1.0f / (pi * pa));
                 cont = true;
                 n_o = make_float4(pos.x, pos.y, pos.z, nsampled);
-                n_d = make_float4(nd.x, nd.y, nd.z, 0.0f);
+                n_d = make_float4(nd.x, nd.y, nd.z, d4.w);
                 n_thr = make_float4(nf.x, nf.y, nf.z, time);
                 n_misc = make_uint4(misc.x, misc.y, uint32_t(pst), uint32_t(pst >> 32));
                 n_extra = make_uint4(uint32_t(nst), uint32_t(nst >> 32), extra.z + 1u, extra.w);
@@ -883,12 +883,11 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
         }
         uint32_t q_pos = block_append(Wv.counters + (cur ^ 1), cont ? 1u : 0u, scan);
         if (cont) {
-            Wv.ray_o[slot] = n_o;
-            Wv.ray_d[slot] = n_d;
-            Wv.thr[slot] = n_thr;
-            Wv.misc[slot] = n_misc;
-            Wv.extra[slot] = n_extra;
-            Wv.q[cur ^ 1][q_pos] = slot;
+            Wv.ray_o[cur ^ 1][q_pos] = n_o;
+            Wv.ray_d[cur ^ 1][q_pos] = n_d;
+            Wv.thr[cur ^ 1][q_pos] = n_thr;
+            Wv.misc[cur ^ 1][q_pos] = n_misc;
+            Wv.extra[cur ^ 1][q_pos] = n_extra;
         }
     }
 }
@@ -1266,14 +1265,13 @@ static uint32_t stack_flags_of(int fmt)
 
 struct WaveBuffers {
     uint32_t cap = 0;
-    float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr;
-    uint4 *misc = nullptr, *extra = nullptr;
+    float4 *ray_o[2] = {}, *ray_d[2] = {}, *thr[2] = {};
+    uint4 *misc[2] = {}, *extra[2] = {};
     float2* hit = nullptr;
-    uint32_t *q0 = nullptr, *q1 = nullptr;
     float4 *sh_o = nullptr, *sh_d = nullptr, *sh_c = nullptr;
     uint8_t* sh_vis = nullptr;
     uint32_t* sh_tag = nullptr;  // allocated by the first guided render
-    uint32_t *job = nullptr, *sh_job = nullptr;  // allocated by the first batched render
+    uint32_t* sh_job = nullptr;  // allocated by the first batched render
     WaveCtl* ctl = nullptr;
     JobDev* d_jobs = nullptr;
     uint32_t jobs_cap = 0;
@@ -1283,9 +1281,11 @@ struct WaveBuffers {
 
     void release()
     {
-        dev_free(ray_o); dev_free(ray_d); dev_free(thr); dev_free(misc); dev_free(extra); dev_free(hit);
-        dev_free(q0); dev_free(q1); dev_free(sh_o); dev_free(sh_d); dev_free(sh_c); dev_free(sh_vis); dev_free(counters);
-        dev_free(sh_tag); dev_free(job); dev_free(sh_job); dev_free(ctl); dev_free(d_jobs); dev_free(cs_bits);
+        for (int b = 0; b < 2; ++b) {
+            dev_free(ray_o[b]); dev_free(ray_d[b]); dev_free(thr[b]); dev_free(misc[b]); dev_free(extra[b]);
+        }
+        dev_free(hit); dev_free(sh_o); dev_free(sh_d); dev_free(sh_c); dev_free(sh_vis); dev_free(counters);
+        dev_free(sh_tag); dev_free(sh_job); dev_free(ctl); dev_free(d_jobs); dev_free(cs_bits);
         cap = 0;
         jobs_cap = 0;
         cs_cap = 0;
@@ -1302,7 +1302,7 @@ struct WaveBuffers {
     int ensure_batch(uint32_t n_jobs)
     {
         int rc;
-        if (!job && cap && ((rc = dev_alloc(&job, cap)) || (rc = dev_alloc(&sh_job, cap)))) return rc;
+        if (!sh_job && cap && (rc = dev_alloc(&sh_job, cap))) return rc;
         if (n_jobs + 1 > jobs_cap) {
             dev_free(d_jobs);
             if ((rc = dev_alloc(&d_jobs, n_jobs + 1))) return rc;
@@ -1320,9 +1320,13 @@ struct WaveBuffers {
         if (n <= cap) return TR_OK;
         release();
         int rc;
-        if ((rc = dev_alloc(&ray_o, n)) || (rc = dev_alloc(&ray_d, n)) || (rc = dev_alloc(&thr, n)) ||
-            (rc = dev_alloc(&misc, n)) || (rc = dev_alloc(&extra, n)) || (rc = dev_alloc(&hit, n)) ||
-            (rc = dev_alloc(&q0, n)) || (rc = dev_alloc(&q1, n)) || (rc = dev_alloc(&sh_o, n)) ||
+        for (int b = 0; b < 2; ++b)
+            if ((rc = dev_alloc(&ray_o[b], n)) || (rc = dev_alloc(&ray_d[b], n)) || (rc = dev_alloc(&thr[b], n)) ||
+                (rc = dev_alloc(&misc[b], n)) || (rc = dev_alloc(&extra[b], n))) {
+                release();
+                return rc;
+            }
+        if ((rc = dev_alloc(&hit, n)) || (rc = dev_alloc(&sh_o, n)) ||
             (rc = dev_alloc(&sh_d, n)) || (rc = dev_alloc(&sh_c, n)) || (rc = dev_alloc(&sh_vis, n)) ||
             (rc = dev_alloc(&counters, C_WORDS)) || (rc = dev_alloc(&ctl, 1))) {
             release();
@@ -1334,9 +1338,12 @@ struct WaveBuffers {
     WaveDev view() const
     {
         WaveDev w;
-        w.ray_o = ray_o; w.ray_d = ray_d; w.thr = thr; w.misc = misc; w.extra = extra; w.hit = hit;
-        w.q[0] = q0; w.q[1] = q1; w.sh_o = sh_o; w.sh_d = sh_d; w.sh_c = sh_c; w.sh_vis = sh_vis; w.sh_tag = sh_tag;
-        w.job = job; w.sh_job = sh_job; w.ctl = ctl;
+        for (int b = 0; b < 2; ++b) {
+            w.ray_o[b] = ray_o[b]; w.ray_d[b] = ray_d[b]; w.thr[b] = thr[b]; w.misc[b] = misc[b]; w.extra[b] = extra[b];
+        }
+        w.hit = hit;
+        w.sh_o = sh_o; w.sh_d = sh_d; w.sh_c = sh_c; w.sh_vis = sh_vis; w.sh_tag = sh_tag;
+        w.sh_job = sh_job; w.ctl = ctl;
         w.counters = counters;
         return w;
     }
