@@ -144,9 +144,11 @@ def bins_sum(x, group=8):
 
 
 def guided_comparison(width=128, height=128, ref_spp=262144, pass_sqrt=16, checkpoints=(4, 8, 16, 32, 64),
-                      train_passes=64, train_sqrt=4, steps_per_wave=8, bsdf_fraction=0.3, learning_rate=3e-3):
+                      train_passes=8, train_sqrt=8, steps_per_wave=64, train_paths=1 << 18, bsdf_fraction=0.3,
+                      learning_rate=3e-3):
     """Guided versus plain rendering of the same film, timed. The guide first trains on
-    `train_passes` passes of train_sqrt^2 spp (one wavefront each, `steps_per_wave` Adam steps after
+    `train_passes` passes of train_sqrt^2 spp (one wavefront each, the first `train_paths` paths of
+    which record training data, `steps_per_wave` Adam steps after
     each); those passes are timed and their film discarded (an untrained network's samples would stay
     in the estimate). The network is then frozen and both estimators render passes of pass_sqrt^2 spp.
 
@@ -200,7 +202,8 @@ def guided_comparison(width=128, height=128, ref_spp=262144, pass_sqrt=16, check
         return out
 
     plain = run(None)
-    guide = renderer.PathGuide(steps_per_wave=steps_per_wave, bsdf_fraction=bsdf_fraction, learning_rate=learning_rate)
+    guide = renderer.PathGuide(steps_per_wave=steps_per_wave, bsdf_fraction=bsdf_fraction, learning_rate=learning_rate,
+                               train_paths=train_paths)
     guided = run(guide)
 
     def time_to(curve, key, target):
@@ -229,7 +232,7 @@ def guided_comparison(width=128, height=128, ref_spp=262144, pass_sqrt=16, check
         "reference": {"spp": n_ref * 1024, "seconds": ref_sec, "estimator": "plain, orthonormal frame"},
         "rel_mse_eps": "1e-2 * mean(reference)^2",
         "guide_settings": {"bsdf_fraction": bsdf_fraction, "learning_rate": learning_rate,
-                           "steps_per_wave": steps_per_wave, "train_passes": train_passes,
+                           "steps_per_wave": steps_per_wave, "train_passes": train_passes, "train_paths": train_paths,
                            "train_pass_spp": train_sqrt ** 2},
         "error_ratio_at_equal_spp": {k: guided[-1][k] / plain[-1][k]
                                      for k in ("rel_mse_trim", "rel_mse_bins8", "rel_mse_steady")},
@@ -393,6 +396,74 @@ def small_jobs_comparison(renderer, n_jobs: int = 256, reps: int = 3) -> dict:
                             "per_job": per_job, "render_calls": calls}}
 
 
+FLOP_PER_EVAL = 35840      # guiding network forward pass per sample (48-64-64-64-64-40, multiply-adds x 2)
+FLOP_PER_TRAIN = 107520    # forward + backward (activation and weight gradients) per training record
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix-core peak
+
+
+def guide_mlp_bench(n_eval: int = 1 << 22, reps: int = 5, train_sizes=(1 << 14, 1 << 18), train_reps: int = 8,
+                    render_passes: int = 4) -> dict:
+    """Throughput of the guiding network's matrix-core kernels: batched evaluation on device
+    buffers (HIP events around the kernel), Adam steps on host-built record batches (wall time per
+    step, record upload included), and guided render passes of the guided workload with a frozen
+    network (the network evaluated inside the shading kernel)."""
+    import ctypes
+    import torch
+    from jarabotransientrendering_amd import renderer, _native as nat
+    rng = np.random.default_rng(0)
+    g = renderer.PathGuide()
+    raw = torch.rand((n_eval, 12), device="cuda", dtype=torch.float32)
+    out = torch.empty((n_eval, 40), device="cuda", dtype=torch.float32)
+    torch.cuda.synchronize()
+    ms = ctypes.c_float()
+    times = []
+    for r in range(reps + 1):
+        nat.check(nat.lib().tr_guide_eval_device(g.handle, ctypes.c_void_p(raw.data_ptr()), n_eval,
+                                                 ctypes.c_void_p(out.data_ptr()), ctypes.byref(ms)))
+        if r:
+            times.append(ms.value)
+    eval_ms = float(np.median(times))
+    eval_tflops = n_eval * FLOP_PER_EVAL / (eval_ms * 1e-3) / 1e12
+    res = {"eval": {"samples": n_eval, "kernel_ms": eval_ms, "tflops": eval_tflops,
+                    "frac_bf16_peak": eval_tflops / BF16_PEAK_TFLOPS}}
+    train = {}
+    for b in train_sizes:
+        gt = renderer.PathGuide(batch_size=b)
+        rec = np.zeros((b, 16), np.float32)
+        rec[:, 0:3] = rng.random((b, 3))
+        rec[:, 3] = rng.random(b)
+        om = rng.normal(size=(b, 3))
+        rec[:, 4:7] = om / np.linalg.norm(om, axis=1, keepdims=True)
+        rec[:, 8:11] = rec[:, 4:7]
+        rec[:, 11] = rng.exponential(1.0, b)
+        om = rng.normal(size=(b, 3))
+        rec[:, 12:15] = om / np.linalg.norm(om, axis=1, keepdims=True)
+        gt.train_batch(rec)
+        t0 = time.perf_counter()
+        for _ in range(train_reps):
+            gt.train_batch(rec)
+        dt = (time.perf_counter() - t0) / train_reps
+        train[str(b)] = {"records": b, "step_ms_wall": dt * 1e3,
+                         "tflops_wall": b * FLOP_PER_TRAIN / dt / 1e12}
+    res["train"] = train
+    # guided passes (frozen network) of the guided workload
+    s = guided_scene(128, 128)
+    world, cam, sampler, film, integ = renderer.setup_scene(s)
+    eng = renderer.TransientRenderer()
+    eng.orthonormal_frame = True
+    eng.profile = True
+    sampler.sqrt_spp = 16
+    g.set_steps(0)
+    kms = {}
+    for k in range(render_passes):
+        st = eng.render(None, world, integ, cam, film, sampler, seed=k, guide=g)
+        for n, v in st["kernel_ms"].items():
+            kms[n] = kms.get(n, 0.0) + v
+    res["guided_render"] = {"passes": render_passes, "paths_per_pass": st["paths"],
+                            "kernel_ms": {k: round(v, 3) for k, v in kms.items()}}
+    return res
+
+
 def copy_probe(nbytes: int, reps: int = 3) -> None:
     """Device-to-device copies of a known size (the HIP runtime's copy kernel), the calibration
     workload for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -425,6 +496,8 @@ def main():
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration)")
+    ap.add_argument("--guide-mlp", action="store_true",
+                    help="only measure the guiding network's kernels (evaluation, training, guided shading)")
     ap.add_argument("--guided", choices=["on", "off", "only"], default="off",
                     help="add the guided-versus-plain time-to-relMSE comparison (only: print just that)")
     args = ap.parse_args()
@@ -438,6 +511,10 @@ def main():
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.guide_mlp:
+        print(json.dumps({"metric": "guiding network throughput", "unit": "TFLOP/s", "dtype": "bf16 x bf16 -> f32",
+                          "peak_tflops": BF16_PEAK_TFLOPS, "data": "synthetic", **guide_mlp_bench()}))
         return
     if args.guided == "only":
         res = guided_comparison()
@@ -521,6 +598,10 @@ def main():
         "path_roofline": {"bytes_per_segment": BYTES_PATH_PER_SEGMENT, "segments_per_path": ext / paths,
                           "achieved": path_gbs, "frac": path_gbs / HBM_PEAK_GBS},
         "kernels_ms": {k: round(v, 3) for k, v in kern_ms.items()},
+        # algorithmic bytes / kernel time for every stage (the film accumulation among them)
+        "kernels_gbs": {k: {"achieved": round(alg_bytes[k] / (kern_ms[k] * 1e-3) / 1e9, 1),
+                            "frac": round(alg_bytes[k] / (kern_ms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                        for k in alg_bytes if kern_ms.get(k)},
         "kernel_time_share": {k: round(v / gpu_time_ms, 4) for k, v in kern_ms.items()} if gpu_time_ms else None,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,

@@ -48,6 +48,7 @@ constexpr int kShadeBlock = 1024;  // shade: one queue atomic per 1024 paths
 constexpr uint32_t kDefaultWave = 1u << 25;
 constexpr uint32_t kFetchMax = 512;  // rays per work fetch of a traversal wavefront (upper bound)
 constexpr uint32_t kTraceRaysPerThread = 16;  // lower bound on a full wavefront's rays per traversal thread
+constexpr uint32_t kTraceMinPerCu = 4;        // traversal blocks per CU below which that bound yields
 
 // Rays per work fetch: large enough that same-address fetch atomics stay rare, small enough that
 // every wavefront of the grid still gets several fetches when the queue is short.
@@ -2138,10 +2139,12 @@ struct Launch {
         const uint32_t wave = c.wave, cu = uint32_t(std::max(compute_units, 1));
         // persistent traversal grid: trace_per_cu blocks per CU, but at least kTraceRaysPerThread rays
         // per thread of a full wavefront (a smaller scene's wavefronts run faster on fewer, longer-lived
-        // blocks: 1.6x on the 2-M-path media workload, no change on 33-M-path wavefronts)
-        const uint32_t tg = std::max(1u, std::min({(wave + kTraceBlock - 1) / kTraceBlock, cu * c.trace_per_cu,
-                                                   (wave + kTraceBlock * kTraceRaysPerThread - 1) /
-                                                       (kTraceBlock * kTraceRaysPerThread)}));
+        // blocks: 1.6x on the 2-M-path media workload, no change on 33-M-path wavefronts) as long as
+        // every CU keeps kTraceMinPerCu blocks (small renders: 128 blocks for a 262,144-path wavefront
+        // left three quarters of the SIMDs idle)
+        const uint32_t by_rays = std::max((wave + kTraceBlock * kTraceRaysPerThread - 1) / (kTraceBlock * kTraceRaysPerThread),
+                                          cu * kTraceMinPerCu);
+        const uint32_t tg = std::max(1u, std::min({(wave + kTraceBlock - 1) / kTraceBlock, cu * c.trace_per_cu, by_rays}));
         const uint32_t fg = std::max(1u, std::min((wave + kFilmRange - 1) / kFilmRange, cu * 4u));
         const uint32_t sblock = c.mode == kShadeGuided ? kShadeBlockGuided : kShadeBlock;
         const uint32_t scap = cu * (c.mode == kShadeGuided ? 8u : c.shade_per_cu);
