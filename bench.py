@@ -143,7 +143,17 @@ def bins_sum(x, group=8):
     return x[:, :, : t // group * group].reshape(r, w, t // group, group, c).sum(axis=3)
 
 
-def guided_comparison(width=128, height=128, ref_spp=262144, pass_sqrt=16, checkpoints=(4, 8, 16, 32, 64),
+GUIDED_WORKLOADS = {
+    # icosphere geometry + the medium, 128x128 px
+    "icosphere_media": dict(ref_sqrt=32, ref_spp=262144, pass_sqrt=16, checkpoints=(4, 8, 16, 32, 64),
+                            train_passes=8, train_sqrt=8, steps_per_wave=64, train_paths=1 << 18),
+    # the 1024-bin media streak film: one 512-pixel scanline of the medium-filled box, point light
+    "media_scanline": dict(ref_sqrt=64, ref_spp=1 << 22, pass_sqrt=64, checkpoints=(4, 8, 16, 32, 64),
+                           train_passes=4, train_sqrt=32, steps_per_wave=32, train_paths=1 << 19),
+}
+
+
+def guided_comparison(name="icosphere_media", ref_sqrt=32, ref_spp=262144, pass_sqrt=16, checkpoints=(4, 8, 16, 32, 64),
                       train_passes=8, train_sqrt=8, steps_per_wave=64, train_paths=1 << 18, bsdf_fraction=0.3,
                       learning_rate=3e-3):
     """Guided versus plain rendering of the same film, timed. The guide first trains on
@@ -161,14 +171,14 @@ def guided_comparison(width=128, height=128, ref_spp=262144, pass_sqrt=16, check
                      per-bin sparsity at low sample counts);
       rel_mse_steady untrimmed, on the time-integrated image."""
     from jarabotransientrendering_amd import renderer
-    s = guided_scene(width, height)
+    s = guided_scene(128, 128) if name == "icosphere_media" else workload("media")[0]
     world, cam, sampler, film, integ = renderer.setup_scene(s)
     eng = renderer.TransientRenderer()
     eng.orthonormal_frame = True
-    # reference: plain passes of 32x32 camera samples until ref_spp
-    sampler.sqrt_spp = 32
+    # reference: plain passes of ref_sqrt^2 camera samples until ref_spp
+    sampler.sqrt_spp = ref_sqrt
     t0 = time.perf_counter()
-    n_ref = max(1, ref_spp // 1024)
+    n_ref = max(1, ref_spp // ref_sqrt ** 2)
     for k in range(n_ref):
         eng.render(None, world, integ, cam, film, sampler, seed=100000 + k)
     ref = film.read().astype(np.float64)
@@ -225,11 +235,15 @@ def guided_comparison(width=128, height=128, ref_spp=262144, pass_sqrt=16, check
         speedups[key] = time_to(plain, key, target) / time_to(guided, key, target)
     key = "rel_mse_trim"
     t_plain, t_guided = time_to(plain, key, plain[-1][key]), time_to(guided, key, plain[-1][key])
+    if name == "icosphere_media":
+        wdesc = (f"{s.n_tris}-triangle displaced icosphere in the open unit box filled with a homogeneous medium "
+                 f"(sigma_a 0.05, sigma_s 0.5), 0.3x0.3 area light; 128x128 px x 1024 bins of 0.01; max 16 events")
+    else:
+        wdesc = ("box filled with a homogeneous isotropic medium (sigma_s 0.5, sigma_a 0.05), point light; "
+                 "scanline 256 of 512x512, 1024 bins of 0.01; max 16 events")
     return {
-        "workload": (f"{s.n_tris}-triangle displaced icosphere in the open unit box filled with a homogeneous "
-                     f"medium (sigma_a 0.05, sigma_s 0.5), 0.3x0.3 area light; {width}x{height} px x 1024 bins "
-                     f"of 0.01; max 16 events; passes of {pass_sqrt ** 2} spp ({pass_paths} paths)"),
-        "reference": {"spp": n_ref * 1024, "seconds": ref_sec, "estimator": "plain, orthonormal frame"},
+        "workload": f"{name}: {wdesc}; passes of {pass_sqrt ** 2} spp ({pass_paths} paths)",
+        "reference": {"spp": n_ref * ref_sqrt ** 2, "seconds": ref_sec, "estimator": "plain, orthonormal frame"},
         "rel_mse_eps": "1e-2 * mean(reference)^2",
         "guide_settings": {"bsdf_fraction": bsdf_fraction, "learning_rate": learning_rate,
                            "steps_per_wave": steps_per_wave, "train_passes": train_passes, "train_paths": train_paths,
@@ -361,6 +375,8 @@ def bench_batch(args) -> dict:
         "small_jobs": small,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+        # against one core of the same CPU baseline (its render-loop rate per process)
+        "gpu_over_cpu_core": (value / cpu["value_per_core"]) if cpu else None,
     }
 
 
@@ -517,10 +533,12 @@ def main():
                           "peak_tflops": BF16_PEAK_TFLOPS, "data": "synthetic", **guide_mlp_bench()}))
         return
     if args.guided == "only":
-        res = guided_comparison()
-        print(json.dumps({"metric": "seconds to target relative MSE, guided vs plain", "value": res["speedup"],
+        res = {k: guided_comparison(k, **v) for k, v in GUIDED_WORKLOADS.items()}
+        print(json.dumps({"metric": "seconds to target relative MSE, guided vs plain",
+                          "value": res["icosphere_media"]["speedup"],
                           "unit": "x (plain time / guided time)", "n_gpus": 1, "higher_is_better": True,
-                          "dtype": "f32 tracer, bf16 network", "data": "synthetic", "guided": res}))
+                          "dtype": "f32 tracer, bf16 network", "data": "synthetic", "guided": res["icosphere_media"],
+                          "guided_media_scanline": res["media_scanline"]}))
         return
     if args.workload == "batch64":
         print(json.dumps(bench_batch(args)))
@@ -605,9 +623,11 @@ def main():
         "kernel_time_share": {k: round(v / gpu_time_ms, 4) for k, v in kern_ms.items()} if gpu_time_ms else None,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+        # against one core of the same CPU baseline (its render-loop rate per process)
+        "gpu_over_cpu_core": (value / cpu["value_per_core"]) if cpu else None,
     }
     if args.guided == "on":
-        out["guided"] = guided_comparison()
+        out["guided"] = {k: guided_comparison(k, **v) for k, v in GUIDED_WORKLOADS.items()}
     print(json.dumps(out))
 
 
