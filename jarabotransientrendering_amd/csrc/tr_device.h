@@ -340,20 +340,34 @@ __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, co
         const int c = int(ref[k] >> 29);
         cna[k] = c == int(kQEmptyDev) ? -1 : c;
     }
+    // Slab distances in ray space: (origin + q * step - o) * inv = q * a + b with a = step * inv
+    // (exact: step is a power of two) and b = (origin - o) * inv, so each plane costs one fused
+    // multiply-add. The entry plane of an axis (the lower one for inv >= 0) is evaluated with
+    // b - e and the exit plane with b + e, e = 3e-7 ((255 step + |origin|) |inv| + |b|), which
+    // bounds the error against the distance of the plane as the host rounds it
+    // (origin + q * step, one rounding): 1.2e-7 |b| (b's two roundings), 6e-8 (255 step +
+    // |origin|) |inv| (the plane's rounding), 6e-8 (|b| + e) (rounding b -+ e) and 6e-8 |t| (the
+    // fused rounding), with |t| <= |b| + 255 step |inv| + e. Entry and exit need no min / max.
+    const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
+    const float bx = (ox - r.o.x) * r.inv.x, by = (oy - r.o.y) * r.inv.y, bz = (oz - r.o.z) * r.inv.z;
+    const float ex = 3e-7f * fmaf(fmaf(255.0f, sx, fabsf(ox)), fabsf(r.inv.x), fabsf(bx));
+    const float ey = 3e-7f * fmaf(fmaf(255.0f, sy, fabsf(oy)), fabsf(r.inv.y), fabsf(by));
+    const float ez = 3e-7f * fmaf(fmaf(255.0f, sz, fabsf(oz)), fabsf(r.inv.z), fabsf(bz));
+    const float bx0 = bx - ex, bx1 = bx + ex, by0 = by - ey, by1 = by + ey, bz0 = bz - ez, bz1 = bz + ez;
+    const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
+    const uint32_t nx = px ? w1.x : w1.y, fx = px ? w1.y : w1.x;  // entry / exit plane bytes
+    const uint32_t ny = py ? w1.z : w1.w, fy = py ? w1.w : w1.z;
+    const uint32_t nz = pz ? w2.x : w2.y, fz = pz ? w2.y : w2.x;
     float key[4];
     int idx[4];
     uint32_t leaves = 0;  // children whose box the ray enters and that are leaves
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        // plane = origin + q * step: q * step is exact, so the fused form rounds once, as the host's
-        const float lxk = fmaf(float((w1.x >> (8 * k)) & 0xFFu), sx, ox), hxk = fmaf(float((w1.y >> (8 * k)) & 0xFFu), sx, ox);
-        const float lyk = fmaf(float((w1.z >> (8 * k)) & 0xFFu), sy, oy), hyk = fmaf(float((w1.w >> (8 * k)) & 0xFFu), sy, oy);
-        const float lzk = fmaf(float((w2.x >> (8 * k)) & 0xFFu), sz, oz), hzk = fmaf(float((w2.y >> (8 * k)) & 0xFFu), sz, oz);
-        float x0 = (lxk - r.o.x) * r.inv.x, x1 = (hxk - r.o.x) * r.inv.x;
-        float y0 = (lyk - r.o.y) * r.inv.y, y1 = (hyk - r.o.y) * r.inv.y;
-        float z0 = (lzk - r.o.z) * r.inv.z, z1 = (hzk - r.o.z) * r.inv.z;
-        float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tnear));
-        float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar)) * 1.00000036f;
+        const float x0 = fmaf(float((nx >> (8 * k)) & 0xFFu), ax, bx0), x1 = fmaf(float((fx >> (8 * k)) & 0xFFu), ax, bx1);
+        const float y0 = fmaf(float((ny >> (8 * k)) & 0xFFu), ay, by0), y1 = fmaf(float((fy >> (8 * k)) & 0xFFu), ay, by1);
+        const float z0 = fmaf(float((nz >> (8 * k)) & 0xFFu), az, bz0), z1 = fmaf(float((fz >> (8 * k)) & 0xFFu), az, bz1);
+        const float t0 = fmaxf(fmaxf(x0, y0), fmaxf(z0, tnear));
+        const float t1 = fminf(fminf(x1, y1), fminf(z1, tfar)) * 1.00000036f;
         bool h = t0 <= t1 && cna[k] >= 0;
         if (h && cna[k] > 0) leaves |= 1u << k;
         h = h && cna[k] == 0;
