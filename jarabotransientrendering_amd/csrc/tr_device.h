@@ -287,10 +287,13 @@ constexpr uint32_t kQEmptyDev = 7u;  // child reference count of an empty slot (
 
 // Per-lane state of a 4-wide BVH query between node steps.
 struct Trav4 {
-    int32_t node;  // node to visit next
-    int32_t sp;    // stack entries
-    int32_t hit;   // closest (or any) hit so far, -1 if none
-    float tfar;    // current far bound (the hit distance once there is a hit)
+    int32_t node;      // node to visit next; while leaves are tested, the node they belong to
+    int32_t sp;        // stack entries
+    int32_t hit;       // closest (or any) hit so far, -1 if none
+    float tfar;        // current far bound (the hit distance once there is a hit)
+    uint32_t tri;      // leaf mode (tri < tri_end): next triangle of the current leaf
+    uint32_t tri_end;  // end of the current leaf's triangles
+    uint32_t leaves;   // leaf mode: entered leaf children of `node` not finished (the lowest bit is current)
 };
 
 __device__ __forceinline__ void trav4_begin(Trav4& st, float tfar)
@@ -299,13 +302,21 @@ __device__ __forceinline__ void trav4_begin(Trav4& st, float tfar)
     st.sp = 0;
     st.hit = -1;
     st.tfar = tfar;
+    st.tri = 0;
+    st.tri_end = 0;
+    st.leaves = 0;
 }
 
-// One node of a closest (ANY = false) or any (ANY = true) hit query over the quantized 4-wide BVH
-// (Bvh4QNodeHost: 4 x 16 bytes per node): the children the ray enters are pushed far-to-near (any hit:
-// in slot order) and the nearest is visited next; entered leaves are tested here. Returns true
-// when the query is complete (st.hit, st.tfar hold the answer). `stack` is the block's LDS stack,
-// `tid` this thread's column and `stride` the block size.
+// One step of a closest (ANY = false) or any (ANY = true) hit query over the quantized 4-wide BVH
+// (Bvh4QNodeHost: 4 x 16 bytes per node). A step visits one node: the inner children the ray
+// enters are pushed far-to-near (any hit: in slot order) and the nearest is visited next. A
+// closest-hit query tests the node's entered leaves in the same step and then drops the inner
+// children behind the hit. An any-hit query instead pushes every entered inner child and tests
+// its leaves one triangle per step (leaf mode), popping the next node when they are done: lanes in
+// leaf mode and lanes visiting nodes share each step, so a wavefront does not wait on a few lanes
+// looping over their leaves (7 % faster shadow rays; for closest hits the lost culling costs
+// more). Returns true when the query is complete (st.hit, st.tfar hold the answer). `stack` is the
+// block's LDS stack, `tid` this thread's column and `stride` the block size.
 template <bool ANY, int SK>
 __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, const float4* __restrict__ tris,
                                             const RayPre& r, float tnear, Trav4& st, uint32_t* stack, int tid,
@@ -326,6 +337,55 @@ __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, co
         }
         ++sp;
     };
+    // next node from the stack; false when the stack is empty (the query is complete). A keyed
+    // entry whose box starts beyond the hit found since it was pushed is dropped (the same
+    // 1 + 2^-21.5 margin as the box test, so no triangle at t <= tfar is skipped).
+    auto pop = [&]() -> bool {
+        while (sp > 0) {
+            --sp;
+            int32_t nd;
+            if constexpr (SK == kStack32) {
+                nd = int32_t(stack[sp * stride + tid]);
+            } else if constexpr (SK == kStack16) {
+                nd = reinterpret_cast<const uint16_t*>(stack)[((sp >> 1) * stride + tid) * 2 + (sp & 1)];
+            } else {
+                const uint32_t e = stack[sp * stride + tid];
+                if (__uint_as_float(e & 0xFFFF0000u) > tfar * 1.00000036f) continue;
+                nd = int32_t(e & 0xFFFFu);
+            }
+            st.node = nd;
+            return true;
+        }
+        return false;
+    };
+    bool done = false;
+    if (ANY && st.tri < st.tri_end) {
+        // leaf mode (any-hit queries): one triangle
+        tri_hit(r, reinterpret_cast<const float*>(tris + 4 * size_t(st.tri)), tnear, tfar, hit);
+        if (ANY && hit >= 0) {
+            st.hit = hit;
+            st.tfar = tfar;
+            return true;
+        }
+        if (++st.tri == st.tri_end) {
+            const uint32_t rest = st.leaves & (st.leaves - 1u);
+            st.leaves = rest;
+            if (rest) {
+                // the next entered leaf of this node: its reference is re-read from the node
+                const uint4 w3 = reinterpret_cast<const uint4*>(nodes)[4 * size_t(st.node) + 3];
+                const int k = __builtin_ctz(rest);
+                const uint32_t ref = uint32_t(sel4(int32_t(w3.x), int32_t(w3.y), int32_t(w3.z), int32_t(w3.w), k));
+                st.tri = ref & 0x1FFFFFFFu;
+                st.tri_end = st.tri + (ref >> 29);
+            } else {
+                done = !pop();
+            }
+        }
+        st.sp = sp;
+        st.hit = hit;
+        st.tfar = tfar;
+        return done;
+    }
     // quantized node (Bvh4QNodeHost): origin, grid exponents, child box bytes, child references
     const uint4* n = reinterpret_cast<const uint4*>(nodes) + 4 * size_t(st.node);
     const uint4 w0 = n[0], w1 = n[1], w2 = n[2], w3 = n[3];
@@ -374,15 +434,15 @@ __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, co
         key[k] = h ? t0 : __int_as_float(0x7f800000);
         idx[k] = h ? cha[k] : -1;
     }
-    // the triangles of every entered leaf, one per step: a wavefront runs as many steps as its
-    // busiest lane needs, not the sum over child slots of each slot's largest leaf
-    if (leaves) {
+    if (!ANY && leaves) {
+        // closest hit: the triangles of every entered leaf now, one per iteration (a wavefront runs
+        // as many iterations as its busiest lane needs), so that the inner children behind a hit
+        // found here are not entered at all
         int k = __builtin_ctz(leaves);
         int i = sel4(cha[0], cha[1], cha[2], cha[3], k);
         int e = i + sel4(cna[0], cna[1], cna[2], cna[3], k);
         for (;;) {
             tri_hit(r, reinterpret_cast<const float*>(tris + 4 * size_t(i)), tnear, tfar, hit);
-            if (ANY && hit >= 0) break;
             if (++i == e) {
                 leaves &= leaves - 1u;
                 if (!leaves) break;
@@ -391,20 +451,12 @@ __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, co
                 e = i + sel4(cna[0], cna[1], cna[2], cna[3], k);
             }
         }
-        if (ANY && hit >= 0) {
-            st.hit = hit;
-            st.tfar = tfar;
-            return true;
-        }
-        if constexpr (!ANY) {
-            // inner children behind a hit found in a sibling leaf are not entered
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                if (!(key[c] <= tfar * 1.00000036f)) {
-                    key[c] = __int_as_float(0x7f800000);
-                    idx[c] = -1;
-                }
-        }
+        for (int c = 0; c < 4; ++c)
+            if (!(key[c] <= tfar * 1.00000036f)) {
+                key[c] = __int_as_float(0x7f800000);
+                idx[c] = -1;
+            }
     }
     if constexpr (!ANY) {
         // sort (key, idx) ascending: 5-comparator network; nearest child first
@@ -424,41 +476,27 @@ __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, co
                 idx[c] = -1;
             }
     }
-    // push the other entered children (far first), descend into the first
+    // push the entered inner children (far first); descend into the nearest, or, for an any-hit
+    // query with entered leaves, push every one of them and test the leaves from the next step on
 #pragma unroll
     for (int k = 3; k >= 1; --k)
         if (idx[k] >= 0) push(idx[k], key[k]);
-    bool done = false;
-    if (idx[0] >= 0) {
+    if (ANY && leaves) {
+        if (idx[0] >= 0) push(idx[0], key[0]);
+        const int k = __builtin_ctz(leaves);
+        st.leaves = leaves;
+        st.tri = uint32_t(sel4(cha[0], cha[1], cha[2], cha[3], k));
+        st.tri_end = st.tri + uint32_t(sel4(cna[0], cna[1], cna[2], cna[3], k));
+    } else if (idx[0] >= 0) {
         st.node = idx[0];
     } else {
-        // pop; a keyed entry whose box starts beyond the hit found since it was pushed is dropped
-        // (the same 1 + 2^-21.5 margin as the box test, so no triangle at t <= tfar is skipped)
-        done = true;
-        while (sp > 0) {
-            --sp;
-            int32_t nd;
-            if constexpr (SK == kStack32) {
-                nd = int32_t(stack[sp * stride + tid]);
-            } else if constexpr (SK == kStack16) {
-                nd = reinterpret_cast<const uint16_t*>(stack)[((sp >> 1) * stride + tid) * 2 + (sp & 1)];
-            } else {
-                const uint32_t e = stack[sp * stride + tid];
-                if (__uint_as_float(e & 0xFFFF0000u) > tfar * 1.00000036f) continue;
-                nd = int32_t(e & 0xFFFFu);
-            }
-            st.node = nd;
-            done = false;
-            break;
-        }
+        done = !pop();
     }
     st.sp = sp;
     st.hit = hit;
     st.tfar = tfar;
     return done;
 }
-
-// A whole query (public closest-hit entry point): node steps until done.
 template <bool ANY, int SK>
 __device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, const float4* __restrict__ tris,
                                           const RayPre& r, float tnear, float& tfar, uint32_t* stack, int tid,
