@@ -895,9 +895,10 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
 
 // ---------------------------------------------------------------------------------------------
 // shadow: occlusion test light -> vertex in (eps, dist - eps]; visible contributions go to the film
-// (8 wavefronts per SIMD: 64 VGPRs, no spills)
+// (7 wavefronts per SIMD: the any-hit leaf mode's state fits 72 VGPRs without spills; at 64 VGPRs
+// it spilled and ran 2 % slower)
 template <int TRAV, int STACK, bool BATCH>
-__global__ void __launch_bounds__(kTraceBlock, 8) shadow_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv)
+__global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv)
 {
     __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
     const uint32_t count = Wv.counters[C_SHADOW];
