@@ -132,6 +132,16 @@ int tr_intersect_batch(tr_scene scene, const tr_ray* rays, uint32_t n, tr_hit* h
 /* Same, on device pointers, asynchronously on `stream` (a hipStream_t; NULL = library stream). */
 int tr_intersect_batch_device(tr_scene scene, const tr_ray* d_rays, uint32_t n, tr_hit* d_hits,
                               void* stream);
+/* Any hit in (tnear, tfar] for each ray: occluded[i] = 1 if some triangle is hit there, else 0 —
+ * World::intersects (src/RayTracing/World.h:617-639 -> rtcIntersect, geomID test) and
+ * World::is_visible (World.h:641-664 -> rtcOccluded; visible = !occluded for the segment
+ * v1 -> v2 with tnear = epsilon, tfar = |v2 - v1| - epsilon). Same watertight test as
+ * tr_intersect_batch, so occluded[i] == (tr_intersect_batch finds a hit) for every ray.
+ * Host pointers; n may be 0. */
+int tr_occluded_batch(tr_scene scene, const tr_ray* rays, uint32_t n, uint8_t* occluded);
+/* Same, on device pointers, asynchronously on `stream` (a hipStream_t; NULL = library stream). */
+int tr_occluded_batch_device(tr_scene scene, const tr_ray* d_rays, uint32_t n, uint8_t* d_occluded,
+                             void* stream);
 
 /* ---- film (StreakCameraFilm) ------------------------------------------------------------- */
 

@@ -130,6 +130,13 @@ public:
         check(tr_intersect_batch(h_, rays.data(), uint32_t(rays.size()), hits.data()), "first_intersection");
         return hits;
     }
+    // batched World::intersects (World.h:617-639): 1 where a triangle is hit in (tnear, tfar]
+    std::vector<uint8_t> intersects(const std::vector<tr_ray>& rays) const
+    {
+        std::vector<uint8_t> occ(rays.size());
+        check(tr_occluded_batch(h_, rays.data(), uint32_t(rays.size()), occ.data()), "intersects");
+        return occ;
+    }
     tr_scene handle() const { return h_; }
 
 private:

@@ -138,6 +138,9 @@ SIGNATURES = {
     "tr_intersect_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
     "tr_intersect_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                  ctypes.c_void_p, ctypes.c_void_p]),
+    "tr_occluded_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "tr_occluded_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                ctypes.c_void_p, ctypes.c_void_p]),
     "tr_film_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(FilmDesc)]),
     "tr_film_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "tr_film_clear": (ctypes.c_int, [ctypes.c_void_p]),

@@ -1148,6 +1148,21 @@ __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, cons
     }
 }
 
+// any hit in (tnear, tfar] for each ray (the shadow kernel's query)
+template <int TRAV, int STACK>
+__global__ void __launch_bounds__(kTraceBlock) occluded_kernel(SceneDev S, const tr_ray* __restrict__ rays,
+                                                               uint8_t* __restrict__ occluded, uint32_t n)
+{
+    __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        tr_ray q = rays[i];
+        RayPre r = ray_pre(mk(q.org[0], q.org[1], q.org[2]), mk(q.dir[0], q.dir[1], q.dir[2]));
+        float tfar = q.tfar;
+        occluded[i] = trace_scene<true, TRAV>(S, r, q.tnear, tfar, stack) >= 0 ? 1 : 0;
+    }
+}
+
 // The traversal stack holds at most one entry per BVH level; the smallest instantiation that
 // covers the scene's depth keeps LDS per block (and so occupancy) in check.
 // `bound` = entries the traversal stack can hold at most (binary: depth; 4-wide: 3 per level).
@@ -1777,6 +1792,45 @@ int tr_intersect_batch(tr_scene s, const tr_ray* rays, uint32_t n, tr_hit* hits)
     if (!out && hipStreamSynchronize(s->stream) != hipSuccess) out = fail(TR_ERR_DEVICE, "tr_intersect_batch: kernel failed");
     dev_free(dr);
     dev_free(dh);
+    return out;
+}
+
+int tr_occluded_batch_device(tr_scene s, const tr_ray* d_rays, uint32_t n, uint8_t* d_occ, void* stream)
+{
+    if (!s) return fail(TR_ERR_INVALID, "tr_occluded_batch_device: null scene");
+    if (!s->committed) return fail(TR_ERR_STATE, "tr_occluded_batch_device: scene not committed");
+    if (n == 0) return TR_OK;
+    if (!d_rays || !d_occ) return fail(TR_ERR_INVALID, "tr_occluded_batch_device: null buffer");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+    const int trav = !s->wide ? kTravBinary : s->n_nodes4 <= 65536u ? kTravWide16 : kTravWide;
+    TR_STACK_DISPATCH(trav, s->wide ? s->stack4 : s->depth, occluded_kernel, trace_grid(n), kTraceBlock, st, s->view(),
+                      d_rays, d_occ, n);
+    HIP_TRY(hipGetLastError());
+    return TR_OK;
+}
+
+int tr_occluded_batch(tr_scene s, const tr_ray* rays, uint32_t n, uint8_t* occluded)
+{
+    if (!s) return fail(TR_ERR_INVALID, "tr_occluded_batch: null scene");
+    if (!s->committed) return fail(TR_ERR_STATE, "tr_occluded_batch: scene not committed");
+    if (n == 0) return TR_OK;
+    if (!rays || !occluded) return fail(TR_ERR_INVALID, "tr_occluded_batch: null buffer");
+    tr_ray* dr = nullptr;
+    uint8_t* dq = nullptr;
+    int rc;
+    if ((rc = dev_alloc(&dr, n)) || (rc = dev_alloc(&dq, n))) {
+        dev_free(dr);
+        return rc;
+    }
+    int out = TR_OK;
+    if (hipMemcpyAsync(dr, rays, size_t(n) * sizeof(tr_ray), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        out = fail(TR_ERR_DEVICE, "tr_occluded_batch: upload failed");
+    if (!out) out = tr_occluded_batch_device(s, dr, n, dq, s->stream);
+    if (!out && hipMemcpyAsync(occluded, dq, size_t(n), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+        out = fail(TR_ERR_DEVICE, "tr_occluded_batch: download failed");
+    if (!out && hipStreamSynchronize(s->stream) != hipSuccess) out = fail(TR_ERR_DEVICE, "tr_occluded_batch: kernel failed");
+    dev_free(dr);
+    dev_free(dq);
     return out;
 }
 

@@ -168,6 +168,23 @@ class World:
         t = hits[:, 0].view(np.float32).copy()
         return t, hits[:, 1].copy(), hits[:, 2].copy(), hits[:, 3].copy()
 
+    def intersects(self, origins: np.ndarray, directions: np.ndarray, max_length=math.inf,
+                   epsilon: float = 1e-5) -> np.ndarray:
+        """Batched World::intersects (World.h:617-639): True where some triangle is hit in
+        (epsilon, max_length - epsilon]; `max_length` may be a scalar or one value per ray, and
+        epsilon defaults to _SIGMA_VISIBILITY_ (1e-5), as in the reference."""
+        o = np.asarray(origins, dtype=np.float32).reshape(-1, 3)
+        d = np.asarray(directions, dtype=np.float32).reshape(-1, 3)
+        n = o.shape[0]
+        rays = np.zeros((n, 8), dtype=np.float32)
+        rays[:, 0:3] = o
+        rays[:, 3] = epsilon
+        rays[:, 4:7] = d
+        rays[:, 7] = np.asarray(max_length, dtype=np.float64) - epsilon
+        occ = np.zeros(n, dtype=np.uint8)
+        nat.check(nat.lib().tr_occluded_batch(self._h, nat.as_ptr(rays), n, nat.as_ptr(occ)))
+        return occ.astype(bool)
+
 
 class PinholeCamera:
     """PinholeCamera (src/Camera/PinholeCamera.h:36-131); `fov_deg` as -camera-fov."""

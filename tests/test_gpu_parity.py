@@ -105,6 +105,52 @@ def test_intersect_matches_reference_fixture(R, tag):
     np.testing.assert_allclose(t[both], ref_t[both], rtol=1e-5, atol=1e-6)
 
 
+def _occluded(w, o, d, tnear, tfar):
+    """tr_occluded_batch on explicit (tnear, tfar) per ray."""
+    from jarabotransientrendering_amd import _native as nat
+    n = len(o)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7], rays[:, 7] = o, tnear, d, tfar
+    occ = np.zeros(n, np.uint8)
+    nat.check(nat.lib().tr_occluded_batch(w._h, nat.as_ptr(rays), n, nat.as_ptr(occ)))
+    return occ.astype(bool)
+
+
+@pytest.mark.parametrize("level", [None, 3, 6])
+def test_occluded_matches_closest_hit(R, oracle, level):
+    """World::intersects (any hit in (tnear, tfar]) against the closest-hit query over the same
+    interval and the oracle's watertight restatement: the same triangle test decides both, so the
+    answers agree ray for ray. Includes axis-parallel and nearly axis-parallel directions (slab
+    reciprocals of 1e20 and 1e7), and intervals that end exactly at the closest hit (closed end)."""
+    s = scenes.cornell_box() if level is None else scenes.icosphere_box(level=level)
+    w, *_ = R.setup_scene(s)
+    o, d = _rays(20000, 31 + (level or 0))
+    rng = np.random.default_rng(41 + (level or 0))
+    k = np.arange(1200)
+    ax = np.zeros((1200, 3), np.float32)
+    ax[k, k % 3] = np.where(k % 2 == 1, 1.0, -1.0)
+    ax[600:, ((k[600:] + 1) % 3)] = np.where(k[600:] % 4 < 2, 1e-7, -1e-7)
+    o = np.concatenate([o, rng.uniform(0.02, 0.98, (1200, 3)).astype(np.float32)])
+    d = np.concatenate([d, ax])
+    tfar = rng.uniform(0.0, 1.6, len(o)).astype(np.float32)
+    occ = _occluded(w, o, d, 1e-5, tfar)
+    t, _, _, tri = w.first_intersection(o, d, tnear=1e-5, tfar=tfar)
+    np.testing.assert_array_equal(occ, tri != 0xFFFFFFFF)
+    wt_ids, wt_t = oracle.intersect(s, np.concatenate([o, d], 1), tnear=1e-5, method=oracle.ISECT_WATERTIGHT)
+    np.testing.assert_array_equal(occ, (wt_ids >= 0) & (wt_t <= tfar))
+    assert 0.2 < occ.mean() < 0.9
+    # intervals ending exactly at the closest hit include it; ending one float below, exclude it
+    t_all, _, _, tri_all = w.first_intersection(o, d, tnear=1e-5)
+    hit = tri_all != 0xFFFFFFFF
+    assert _occluded(w, o[hit], d[hit], 1e-5, t_all[hit]).all()
+    below = np.nextafter(t_all[hit], np.float32(0.0))
+    assert not _occluded(w, o[hit], d[hit], 1e-5, below).any()
+    # the Python mirror: World.intersects(max_length, epsilon) tests (epsilon, max_length - epsilon]
+    m = w.intersects(o[:2000], d[:2000], max_length=tfar[:2000].astype(np.float64) + 1e-5)
+    np.testing.assert_array_equal(m, _occluded(w, o[:2000], d[:2000], 1e-5,
+                                               (tfar[:2000].astype(np.float64) + 1e-5 - 1e-5).astype(np.float32)))
+
+
 def test_intersect_interval_ids_and_misses(R):
     s = scenes.cornell_box()
     w, *_ = R.setup_scene(s)
