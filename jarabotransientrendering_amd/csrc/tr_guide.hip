@@ -297,13 +297,39 @@ __global__ void __launch_bounds__(64) train_kernel(TrainArgs A)
     }
 }
 
+// First stage of the gradient sum: block row y adds the per-chunk gradients of its slice of the
+// step's chunks, in chunk order, so the sum is deterministic and the Adam kernel adds only
+// kReduceSlices values per parameter (it summed every chunk serially, 256 for 16,384 records).
+constexpr uint32_t kReduceSlices = 32;
+
+__global__ void reduce_kernel(const float* __restrict__ partial, const float* __restrict__ partial_loss,
+                              const uint32_t* count, uint32_t first, uint32_t batch, float* __restrict__ part2,
+                              float* __restrict__ loss2)
+{
+    const uint32_t total = *count;
+    const uint32_t avail = total > first ? min(total - first, batch) : 0u;
+    const uint32_t chunks = (avail + 63u) / 64u;
+    const uint32_t per = (chunks + kReduceSlices - 1u) / kReduceSlices;
+    const uint32_t c0 = min(blockIdx.y * per, chunks), c1 = min(c0 + per, chunks);
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        float ls = 0.f;
+        for (uint32_t c = c0; c < c1; ++c) ls += partial_loss[c];
+        loss2[blockIdx.y] = ls;
+    }
+    if (p >= kParams) return;
+    float sum = 0.f;
+    for (uint32_t c = c0; c < c1; ++c) sum += partial[size_t(c) * kParams + p];
+    part2[size_t(blockIdx.y) * kParams + p] = sum;
+}
+
 // Adam (Kingma & Ba 2015) on the master FP32 parameters: gradient = mean over the step's records.
 struct AdamArgs {
     float* master;
     float* m;
     float* v;
-    const float* partial;
-    const float* partial_loss;
+    const float* partial;       // [kReduceSlices][kParams] slice sums (reduce_kernel)
+    const float* partial_loss;  // [kReduceSlices]
     const uint32_t* count;
     uint32_t first, batch;
     float lr, beta1, beta2, eps;
@@ -316,16 +342,15 @@ __global__ void adam_kernel(AdamArgs A)
     const uint32_t total = *A.count;
     const uint32_t avail = total > A.first ? min(total - A.first, A.batch) : 0u;
     if (avail == 0) return;
-    const uint32_t chunks = (avail + 63u) / 64u;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
         float ls = 0.f;
-        for (uint32_t c = 0; c < chunks; ++c) ls += A.partial_loss[c];
+        for (uint32_t c = 0; c < kReduceSlices; ++c) ls += A.partial_loss[c];
         *A.loss_out = ls / float(avail);
     }
     if (p >= kParams) return;
     float gsum = 0.f;
-    for (uint32_t c = 0; c < chunks; ++c) gsum += A.partial[size_t(c) * kParams + p];
+    for (uint32_t c = 0; c < kReduceSlices; ++c) gsum += A.partial[size_t(c) * kParams + p];
     float gr = gsum / float(avail);
     float m = A.beta1 * A.m[p] + (1.0f - A.beta1) * gr;
     float v = A.beta2 * A.v[p] + (1.0f - A.beta2) * gr * gr;
@@ -435,6 +460,7 @@ struct tr_guide_s {
     float *master = nullptr, *m = nullptr, *v = nullptr, *bias = nullptr;
     uint32_t *fwd = nullptr, *bwd = nullptr;
     float *partial = nullptr, *partial_loss = nullptr;
+    float *part2 = nullptr, *loss2 = nullptr;  // slice sums of the per-chunk gradients
     uint32_t max_chunks = 0;
     // render-time buffers
     float4* rec = nullptr;
@@ -459,7 +485,7 @@ struct tr_guide_s {
     {
         release_render();
         gfree(master), gfree(m), gfree(v), gfree(bias), gfree(fwd), gfree(bwd), gfree(partial), gfree(partial_loss);
-        gfree(count), gfree(wsum);
+        gfree(part2), gfree(loss2), gfree(count), gfree(wsum);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -490,7 +516,10 @@ struct tr_guide_s {
             GHIP_TRY(hipGetLastError());
             ++steps;
             AdamArgs A;
-            A.master = master, A.m = m, A.v = v, A.partial = partial, A.partial_loss = partial_loss;
+            reduce_kernel<<<dim3((kParams + 255) / 256, kReduceSlices), 256, 0, s>>>(partial, partial_loss, count, T.first,
+                                                                                     T.batch, part2, loss2);
+            GHIP_TRY(hipGetLastError());
+            A.master = master, A.m = m, A.v = v, A.partial = part2, A.partial_loss = loss2;
             A.count = count, A.first = T.first, A.batch = T.batch;
             A.lr = desc.learning_rate, A.beta1 = 0.9f, A.beta2 = 0.999f, A.eps = 1e-8f;
             A.bc1 = float(1.0 - std::pow(0.9, double(steps)));
@@ -601,6 +630,7 @@ int tr_guide_create(tr_guide* out, const tr_guide_desc* d)
         (rc = galloc(&g->bias, kBiasPad)) || (rc = galloc(&g->fwd, size_t(kFwdFrags) * 256)) ||
         (rc = galloc(&g->bwd, size_t(kBwdFrags) * 256)) ||
         (rc = galloc(&g->partial, size_t(g->max_chunks) * kParams)) || (rc = galloc(&g->partial_loss, g->max_chunks)) ||
+        (rc = galloc(&g->part2, size_t(kReduceSlices) * kParams)) || (rc = galloc(&g->loss2, kReduceSlices)) ||
         (rc = galloc(&g->count, 4)) || (rc = galloc(&g->wsum, 4)))
         return bail(rc);
     std::vector<float> p = initial_params(D.init_seed);
