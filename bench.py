@@ -398,7 +398,8 @@ def bench_batch(args) -> dict:
 
 def small_jobs_comparison(renderer, n_jobs: int = 256, reps: int = 3) -> dict:
     """Many small jobs (icosphere level 3-4 in the box, 32x32 px, 128 bins, 16 spp, max 8 bounces):
-    one shared sequence of wavefronts against one job after another and one render call per job."""
+    one shared sequence of wavefronts against one render call per job (the per-job batch mode
+    shares wavefronts among jobs this small, so it is not listed)."""
     scs = []
     for k in range(n_jobs):
         s = scenes.icosphere_box(level=3 + k % 2, width=32, height=32, time_bins=128, exposure=0.08, sqrt_spp=4,
@@ -420,12 +421,11 @@ def small_jobs_comparison(renderer, n_jobs: int = 256, reps: int = 3) -> dict:
     eng = renderer.TransientRenderer()
     shared = rate(lambda: b.render({"flags": 0}))
     shared_graph = rate(lambda: b.render({"flags": 8}))
-    per_job = rate(lambda: b.render({"flags": 128}))
     calls = rate(lambda: [eng.render(None, w, integ, cam, film, sampler, seed=s.render.seed)
                           for (w, cam, sampler, film, integ), s in zip(setups, scs)])
     return {"jobs": n_jobs, "paths_per_render": paths,
             "paths_per_s": {"shared_wavefronts": shared, "shared_wavefronts_graph": shared_graph,
-                            "per_job": per_job, "render_calls": calls}}
+                            "render_calls": calls}}
 
 
 FLOP_PER_EVAL = 35840      # guiding network forward pass per sample (48-64-64-64-64-40, multiply-adds x 2)

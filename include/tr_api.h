@@ -227,8 +227,10 @@ enum {
      * current hit are dropped without visiting them. */
     TR_RENDER_STACK32 = 32u,
     TR_RENDER_STACK_KEYED = 64u,
-    /* batches: render the jobs one after another (each with its own wavefronts, single-job kernels
-     * and the LDS-privatised film) instead of in one shared sequence; tr_autotune tries both */
+    /* batches: render each job of at least 2^22 paths on its own (its own wavefronts, single-job
+     * kernels and the LDS-privatised film), one after another, and the smaller jobs together in one
+     * shared sequence of wavefronts; without the flag every job shares one sequence. tr_autotune
+     * tries both */
     TR_RENDER_PER_JOB = 128u
 };
 

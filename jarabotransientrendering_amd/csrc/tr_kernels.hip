@@ -2723,16 +2723,27 @@ int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
 
 namespace {
 
+// Per-job mode: jobs of at least kPerJobMinPaths paths (enough to fill the GPU on their own) render
+// one after another with the single-job kernels and LDS-privatised film; the smaller ones share one
+// sequence of wavefronts, where launch overheads are paid once for all of them.
+constexpr uint64_t kPerJobMinPaths = 1ull << 22;
+
 int run_batch(tr_batch_s* b, std::vector<HostJob>& jobs, const LaunchCfg& c, tr_render_stats* stats,
               std::chrono::steady_clock::time_point t0)
 {
     if (!c.per_job || jobs.size() == 1)
         return run_jobs(b->stream, b->wave, jobs, c, nullptr, b->compute_units, b->ev, &b->ev_made, stats, t0);
-    tr_render_stats total{};
+    std::vector<std::vector<HostJob>> groups;
+    std::vector<HostJob> small;
     for (auto& j : jobs) {
-        std::vector<HostJob> one(1, j);
+        if (j.paths >= kPerJobMinPaths) groups.emplace_back(1, j);
+        else small.push_back(j);
+    }
+    if (!small.empty()) groups.push_back(std::move(small));
+    tr_render_stats total{};
+    for (auto& g : groups) {
         tr_render_stats st{};
-        int rc = run_jobs(b->stream, b->wave, one, c, nullptr, b->compute_units, b->ev, &b->ev_made, &st, t0);
+        int rc = run_jobs(b->stream, b->wave, g, c, nullptr, b->compute_units, b->ev, &b->ev_made, &st, t0);
         if (rc) return rc;
         total.paths += st.paths;
         total.extension_rays += st.extension_rays;

@@ -87,9 +87,13 @@ def test_graph_replay_equals_direct_launches(R):
 
 
 def test_per_job_mode_equals_single_renders(R):
-    """TR_RENDER_PER_JOB renders the jobs one after another; the films and totals are unchanged."""
+    """TR_RENDER_PER_JOB renders the jobs of at least 2^22 paths one after another and the smaller
+    ones together; the films and totals are unchanged."""
     from jarabotransientrendering_amd import _native as nat
-    scs = _jobs()
+    big = scenes.icosphere_box(level=2, width=64, height=64, time_bins=128, sqrt_spp=32, n_incoming=1,
+                               max_bounces=4, seed=17)
+    assert big.n_paths() >= 1 << 22
+    scs = _jobs()[:2] + [big] + _jobs()[2:]
     imgs, st, _ = _batch(R, scs, launch={"flags": nat.TR_RENDER_PER_JOB})
     ref = [_single(R, s) for s in scs]
     assert st["paths"] == sum(s.n_paths() for s in scs)
