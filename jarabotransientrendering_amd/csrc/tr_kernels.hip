@@ -113,7 +113,10 @@ struct RenderDev {
 };
 
 struct WaveDev {
-    // path records of the current ([cur]) and the next ([cur ^ 1]) bounce
+    // path records of the current ([cur]) and the next ([cur ^ 1]) bounce. The camera kernel writes
+    // no origin and no throughput: every camera ray starts at the camera with unit throughput at
+    // time 0, so the first bounce takes them from the camera set-up (and the sampled free-flight
+    // distance from extra.w, which holds the unwarp distance only from the first hit on).
     float4* ray_o[2]; // origin xyz, w = sampled free-flight distance of this segment (inf without medium)
     float4* ray_d[2]; // direction xyz, w = batched renders: job (uint bits); otherwise the path's
                       // index in its wavefront (uint bits)
@@ -129,6 +132,7 @@ struct WaveDev {
     uint32_t* sh_job; // batched renders: job of each shadow record
     struct WaveCtl* ctl;
     uint32_t* counters;
+    float3 cpos;      // single-job renders: camera position (batched renders: their job's R.cpos)
 };
 
 // One render job of a batched launch: its scene, film/camera set-up and first global path index.
@@ -236,11 +240,9 @@ __global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S0, RenderDev R
     if (S.has_medium) {
         sampled = free_flight(pcg_float(pst, pinc), R.time_sampling ? R.ts_lambda : S.lambda);
     }
-    Wv.ray_o[0][slot] = make_float4(R.cpos.x, R.cpos.y, R.cpos.z, sampled);
     Wv.ray_d[0][slot] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(BATCH ? jid : slot));
-    Wv.thr[0][slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     Wv.misc[0][slot] = make_uint4(uint32_t(pid), uint32_t(pid >> 32), uint32_t(pst), uint32_t(pst >> 32));
-    Wv.extra[0][slot] = make_uint4(uint32_t(nst), uint32_t(nst >> 32), pix_local << 8, __float_as_uint(3.402823466e+38f));
+    Wv.extra[0][slot] = make_uint4(uint32_t(nst), uint32_t(nst >> 32), pix_local << 8, __float_as_uint(sampled));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -334,9 +336,11 @@ struct RayRefill {
 // scene pointers fit the 72-VGPR budget without spills)
 template <int TRAV, int STACK, bool BATCH>
 __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv,
-                                                             int cur)
+                                                             int cur_first)
 {
     __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
+    const int cur = cur_first & 1;
+    const bool first = (cur_first & 2) != 0;  // camera rays: the origin is the camera's
     const uint32_t count = Wv.counters[cur];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
@@ -357,12 +361,15 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
         for (;;) {
             rf.run(count, chunk, Wv.counters + C_FETCH_EXT, busy, [&](uint32_t i) {
                 rec = i;
-                const float4 o = Wv.ray_o[cur][i], d = Wv.ray_d[cur][i];
+                const float4 d = Wv.ray_d[cur][i];
+                float3 c = Wv.cpos;
                 if constexpr (BATCH) {
-                    const SceneDev& S = jobs[__float_as_uint(d.w)].S;
-                    nodes = S.nodes4;
-                    tris = S.tris;
+                    const JobDev& J = jobs[__float_as_uint(d.w)];
+                    nodes = J.S.nodes4;
+                    tris = J.S.tris;
+                    c = J.R.cpos;
                 }
+                const float4 o = first ? make_float4(c.x, c.y, c.z, 0.0f) : Wv.ray_o[cur][i];
                 r = ray_pre(mk(o), mk(d));
                 trav4_begin(st, __int_as_float(0x7f800000));
             });
@@ -385,8 +392,10 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
             for (uint32_t j = 0; j < chunk; j += 64) {
                 const uint32_t i = base + j + lane;
                 if (i < count) {
-                    float4 o = Wv.ray_o[cur][i], d = Wv.ray_d[cur][i];
+                    const float4 d = Wv.ray_d[cur][i];
                     const SceneDev& S = BATCH ? jobs[__float_as_uint(d.w)].S : S0;
+                    const float3 c = BATCH ? jobs[__float_as_uint(d.w)].R.cpos : Wv.cpos;
+                    const float4 o = first ? make_float4(c.x, c.y, c.z, 0.0f) : Wv.ray_o[cur][i];
                     RayPre r = ray_pre(mk(o), mk(d));
                     float tfar = __int_as_float(0x7f800000);
                     int32_t tri = trace_scene<false, TRAV>(S, r, kSigmaVisibility, tfar, stack);
@@ -608,12 +617,16 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
         uint64_t pst = 0, nst = 0, pinc = 0;
         uint4 misc = {}, extra = {};
         if (live) {
-            float4 o4 = Wv.ray_o[cur][i], thr = Wv.thr[cur][i];
             misc = Wv.misc[cur][i];
             extra = Wv.extra[cur][i];
             float2 h = Wv.hit[i];
             int32_t tri = __float_as_int(h.y);
             uint32_t level = extra.z & 0xFFu;
+            // a camera ray starts at the camera with unit throughput; extra.w holds its sampled
+            // free-flight distance until the unwarp distance replaces it below
+            const float4 o4 = level == 0 ? make_float4(R.cpos.x, R.cpos.y, R.cpos.z, __uint_as_float(extra.w))
+                                         : Wv.ray_o[cur][i];
+            const float4 thr = level == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : Wv.thr[cur][i];
             f3 o = mk(o4);
             d = mk(d4);
             float sampled = o4.w;
@@ -2196,7 +2209,8 @@ struct Launch {
         const RenderDev R0 = j0.dev.R;
         const JobDev* dj = batch ? wb->d_jobs : nullptr;
         const uint32_t nj = uint32_t(jobs->size());
-        const WaveDev Wv = wb->view();
+        WaveDev Wv = wb->view();
+        Wv.cpos = R0.cpos;
         const uint32_t wave = c.wave, cu = uint32_t(std::max(compute_units, 1));
         // persistent traversal grid: trace_per_cu blocks per CU, but at least kTraceRaysPerThread rays
         // per thread of a full wavefront (a smaller scene's wavefronts run faster on fewer, longer-lived
@@ -2218,7 +2232,8 @@ struct Launch {
         int cur = 0;
         for (int b = 0; b < bounces; ++b) {
             if ((rc = begin(TR_K_EXTEND))) return rc;
-            TR_TRACE_DISPATCH(trav_ext, stack_bound, batch, extend_kernel, tg, kTraceBlock, stream, S0, dj, Wv, cur);
+            TR_TRACE_DISPATCH(trav_ext, stack_bound, batch, extend_kernel, tg, kTraceBlock, stream, S0, dj, Wv,
+                              cur | (b == 0 ? 2 : 0));
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADE))) return rc;
             if (batch) {
