@@ -344,7 +344,7 @@ def bench_batch(args) -> dict:
            "shadow": BYTES_SHADOW_PER_RAY * prof["shadow_rays"],
            "shade": BYTES_SHADE_PER_SEGMENT * prof["extension_rays"],
            "film": BYTES_FILM_PER_RECORD * prof["shadow_rays"] + BYTES_FILM_PER_CONTRIB * prof["contributions"],
-           "camera": 80 * prof["paths"],
+           "camera": 48 * prof["paths"],
            "film_merge": sum(2 * 4 * f.rows * f.width * f.time_bins * 3 for (_, _, _, f, _) in setups)}
     dom = max(kern_ms, key=lambda n: kern_ms[n])
     avg_ms = kern_ms[dom] / max(kern_n[dom], 1)
@@ -568,7 +568,10 @@ def main():
     world, cam, sampler, film, integ = renderer.setup_scene(scene)
     eng = renderer.TransientRenderer()
     eng.wave_paths = args.wave
-    eng.profile = True
+    # the timed steps carry HIP events around the dominant kernel (extend) only: an event pair
+    # between two launches drains the GPU, and timing every kernel would add 2-14 % to a step
+    eng.profile = False
+    eng.extra_flags = nat.TR_RENDER_PROFILE_EXTEND
     eng.binary_bvh = args.binary_bvh
     for k in range(args.warmup):
         eng.render(None, world, integ, cam, film, sampler, seed=1000 + k)
@@ -580,9 +583,17 @@ def main():
     wall = time.perf_counter() - t0
     paths = sum(s["paths"] for s in steps)
     value = paths / wall
-
-    kern_ms = {n: sum(s["kernel_ms"][n] for s in steps) for n in steps[0]["kernel_ms"]}
-    kern_n = {n: sum(s["kernel_launches"][n] for s in steps) for n in steps[0]["kernel_launches"]}
+    dom = "extend"
+    dom_ms = sum(s["kernel_ms"][dom] for s in steps)
+    dom_n = sum(s["kernel_launches"][dom] for s in steps)
+    # per-kernel split: the same steps again, every kernel timed (after the timed region)
+    eng.profile = True
+    eng.extra_flags = 0
+    t1 = time.perf_counter()
+    psteps = [eng.render(None, world, integ, cam, film, sampler, seed=k) for k in range(args.steps)]
+    pwall = time.perf_counter() - t1
+    kern_ms = {n: sum(s["kernel_ms"][n] for s in psteps) for n in psteps[0]["kernel_ms"]}
+    kern_n = {n: sum(s["kernel_launches"][n] for s in psteps) for n in psteps[0]["kernel_launches"]}
     ext = sum(s["extension_rays"] for s in steps)
     shd = sum(s["shadow_rays"] for s in steps)
     con = sum(s["contributions"] for s in steps)
@@ -592,11 +603,10 @@ def main():
         "film": BYTES_FILM_PER_RECORD * shd + BYTES_FILM_PER_CONTRIB * con,
         "shade": BYTES_SHADE_PER_SEGMENT * ext,
         "film_merge": 2 * 4 * film.rows * film.width * film.time_bins * 3 * args.steps,
-        "camera": 80 * paths,
+        "camera": 48 * paths,  # direction, path id + stream state, light stream + pixel + free flight
     }
-    dom = max(kern_ms, key=lambda n: kern_ms[n])
-    avg_ms = kern_ms[dom] / max(kern_n[dom], 1)
-    bytes_per_launch = alg_bytes[dom] / max(kern_n[dom], 1)
+    avg_ms = dom_ms / max(dom_n, 1)
+    bytes_per_launch = alg_bytes[dom] / max(dom_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
     try:
@@ -627,11 +637,13 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "avg_launch_ms": avg_ms, "launches": kern_n[dom],
+                     "avg_launch_ms": avg_ms, "launches": dom_n,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "path_roofline": {"bytes_per_segment": BYTES_PATH_PER_SEGMENT, "segments_per_path": ext / paths,
                           "achieved": path_gbs, "frac": path_gbs / HBM_PEAK_GBS},
+        # per-kernel split from the same number of further steps with every kernel timed
         "kernels_ms": {k: round(v, 3) for k, v in kern_ms.items()},
+        "kernels_ms_step_ms": pwall / args.steps * 1e3,
         # algorithmic bytes / kernel time for every stage (the film accumulation among them)
         "kernels_gbs": {k: {"achieved": round(alg_bytes[k] / (kern_ms[k] * 1e-3) / 1e9, 1),
                             "frac": round(alg_bytes[k] / (kern_ms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}

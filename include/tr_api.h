@@ -227,6 +227,10 @@ enum {
      * current hit are dropped without visiting them. */
     TR_RENDER_STACK32 = 32u,
     TR_RENDER_STACK_KEYED = 64u,
+    /* TR_RENDER_PROFILE for the closest-hit (extend) launches only: kernel_ms of the other kinds
+     * stays 0. Each event pair between two launches drains the GPU, so timing every kernel adds
+     * 2-14 % to a render; timing one kind adds a fraction of that. */
+    TR_RENDER_PROFILE_EXTEND = 256u,
     /* batches: render each job of at least 2^22 paths on its own (its own wavefronts, single-job
      * kernels and the LDS-privatised film), one after another, and the smaller jobs together in one
      * shared sequence of wavefronts; without the flag every job shares one sequence. tr_autotune

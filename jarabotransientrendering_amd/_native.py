@@ -20,6 +20,7 @@ TR_ERR_STATE = -4
 
 TR_MATERIAL_TWO_SIDED = 1
 TR_RENDER_PROFILE = 1
+TR_RENDER_PROFILE_EXTEND = 256
 TR_RENDER_BINARY_BVH = 2
 TR_RENDER_ORTHONORMAL_FRAME = 4
 TR_RENDER_GRAPH = 8

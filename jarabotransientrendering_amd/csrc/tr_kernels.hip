@@ -1280,6 +1280,7 @@ struct LaunchCfg {
     bool wide = true;               // 4-wide BVH
     bool graph = false;             // replay each wavefront's bounce loop from a captured hipGraph
     bool prof = false;              // per-kernel HIP events
+    bool prof_extend = false;       // ... around the extend launches only
     int mode = kShadeRef;
     // 4-wide traversal stack entries: 0 = 16-bit when the BVH has at most 65,536 nodes (else
     // 32-bit); 1 = always 32-bit; 2 = keyed 16-bit for closest hits (16-bit for any hits)
@@ -2165,6 +2166,8 @@ struct Launch {
     double kernel_ms[TR_K_COUNT] = {};
     uint32_t launches[TR_K_COUNT] = {};
     bool events_on = false;
+    bool extend_only = false;  // events around the extend launches only
+    bool open = false;         // begin() recorded an event that end() closes
 
     int flush_events()
     {
@@ -2181,7 +2184,8 @@ struct Launch {
     int begin(int kind)
     {
         launches[kind]++;
-        if (!events_on) return TR_OK;
+        open = events_on && !(extend_only && kind != TR_K_EXTEND);
+        if (!open) return TR_OK;
         if (ev_kind.size() == 256) {
             int r2 = flush_events();
             if (r2) return r2;
@@ -2194,7 +2198,8 @@ struct Launch {
     int end()
     {
         HIP_TRY(hipGetLastError());
-        if (!events_on) return TR_OK;
+        if (!open) return TR_OK;
+        open = false;
         HIP_TRY(hipEventRecord(ev[2 * ev_kind.back().second + 1], stream));
         return TR_OK;
     }
@@ -2351,6 +2356,7 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
             *ev_made = true;
         }
         L.events_on = true;
+        L.extend_only = c.prof_extend;
     }
     // LDS histogram slots of the film kernel: as many pixels x T bins x RGB as fit in kFilmLdsBytes
     {
@@ -2469,7 +2475,8 @@ int render_impl(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_des
     if (d->wave_paths) c.wave = d->wave_paths;
     c.wide = (d->flags & TR_RENDER_BINARY_BVH) == 0;
     c.graph = (d->flags & TR_RENDER_GRAPH) != 0;
-    c.prof = (d->flags & TR_RENDER_PROFILE) != 0;
+    c.prof = (d->flags & (TR_RENDER_PROFILE | TR_RENDER_PROFILE_EXTEND)) != 0;
+    c.prof_extend = (d->flags & TR_RENDER_PROFILE) == 0 && (d->flags & TR_RENDER_PROFILE_EXTEND) != 0;
     if (d->flags & (TR_RENDER_STACK32 | TR_RENDER_STACK_KEYED)) c.stack_fmt = stack_fmt_of(d->flags);
     c.mode = guide ? kShadeGuided : ((d->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef);
     return run_jobs(s->stream, s->wave, jobs, c, guide, s->compute_units, s->ev, &s->ev_made, stats, t_start);
@@ -2535,7 +2542,8 @@ LaunchCfg launch_cfg(const tr_batch_s* b, const tr_launch_desc* L)
         if (L->shade_per_cu) c.shade_per_cu = L->shade_per_cu;
         c.wide = (L->flags & TR_RENDER_BINARY_BVH) == 0;
         c.graph = (L->flags & TR_RENDER_GRAPH) != 0;
-        c.prof = (L->flags & TR_RENDER_PROFILE) != 0;
+        c.prof = (L->flags & (TR_RENDER_PROFILE | TR_RENDER_PROFILE_EXTEND)) != 0;
+        c.prof_extend = (L->flags & TR_RENDER_PROFILE) == 0 && (L->flags & TR_RENDER_PROFILE_EXTEND) != 0;
         c.mode = (L->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef;
         c.stack_fmt = stack_fmt_of(L->flags);
         c.per_job = (L->flags & TR_RENDER_PER_JOB) != 0;
