@@ -38,6 +38,25 @@ def test_python_binding_covers_the_header():
     assert sorted(_native.SIGNATURES) == declared_functions()
 
 
+def test_python_constants_match_the_header():
+    """Every TR_* enumerator and #define the header gives a value has the same value in the binding."""
+    from jarabotransientrendering_amd import _native as nat
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    values = {}
+    for body in re.findall(r"enum\s*\{(.*?)\}", text, flags=re.S):
+        for name, val in re.findall(r"(TR_\w+)\s*=\s*(-?\w+)", body):
+            values[name] = int(val.rstrip("uU"), 0)
+    for name, val in re.findall(r"#define\s+(TR_\w+)\s+\(?(-?\d+)[uU]?\)?", text):
+        values[name] = int(val)
+    assert len(values) >= 15
+    for name, val in values.items():
+        if hasattr(nat, name):
+            assert getattr(nat, name) == val, name
+    for name in ("TR_RENDER_PROFILE", "TR_RENDER_PROFILE_EXTEND", "TR_RENDER_PER_JOB", "TR_RENDER_TIME_SAMPLING",
+                 "TR_RENDER_STACK32", "TR_RENDER_STACK_KEYED", "TR_RENDER_GRAPH"):
+        assert name in values and getattr(nat, name) == values[name]
+
+
 def test_struct_layouts_match_the_header():
     from jarabotransientrendering_amd import _native as nat
     assert ctypes.sizeof(nat.Ray) == 32

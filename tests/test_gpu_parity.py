@@ -298,6 +298,24 @@ def test_edge_cases(R):
     assert img.sum() == 0
 
 
+def test_extend_only_profiling(R):
+    """TR_RENDER_PROFILE_EXTEND times the extend launches alone; the film and the statistics are those
+    of an unprofiled render."""
+    from jarabotransientrendering_amd import _native as nat
+    s = scenes.cornell_box(width=16, height=12, time_bins=64, sqrt_spp=4, n_incoming=2, max_bounces=4, seed=5)
+    a, sa, _ = R.render_scene(s)
+    w, cam, sampler, film, integ = R.setup_scene(s)
+    eng = R.TransientRenderer()
+    eng.extra_flags = nat.TR_RENDER_PROFILE_EXTEND
+    st = eng.render(None, w, integ, cam, film, sampler, seed=s.render.seed)
+    b = film.read()
+    assert st["kernel_ms"]["extend"] > 0
+    assert all(v == 0 for k, v in st["kernel_ms"].items() if k != "extend")
+    assert st["kernel_launches"]["extend"] == sa["kernel_launches"]["extend"] > 0
+    assert st["extension_rays"] == sa["extension_rays"] and st["contributions"] == sa["contributions"]
+    np.testing.assert_allclose(b, a, rtol=2e-5, atol=1e-7)
+
+
 def test_errors_are_reported(R):
     from jarabotransientrendering_amd import _native as nat
     w = R.World()
