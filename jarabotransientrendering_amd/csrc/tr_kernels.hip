@@ -1131,6 +1131,43 @@ __global__ void __launch_bounds__(kBlock) film_merge_kernel(float4* __restrict__
     }
 }
 
+// film read-back: one block per pixel writes (sum + comp) / weight of its T x RGB bins (0/0 -> 0,
+// ImageRow::weight, Image.h:153-162) in double precision, and the pixel's steady value (the sum
+// over its bins) for the steady image (StreakCameraFilm.h:369-397)
+constexpr int kNormBlock = 256;
+__global__ void __launch_bounds__(kNormBlock) film_normalize_kernel(const float* __restrict__ sum,
+                                                                   const float* __restrict__ comp,
+                                                                   const double* __restrict__ weight, uint32_t T3,
+                                                                   float* __restrict__ out, float* __restrict__ steady)
+{
+    __shared__ double part[kNormBlock / 64][3];
+    const size_t p = blockIdx.x;
+    const double w = weight[p];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (uint32_t j = threadIdx.x; j < T3; j += kNormBlock) {
+        const size_t i = p * T3 + j;
+        double v = (double(sum[i]) + double(comp[i])) / w;
+        if (!isfinite(v)) v = 0.0;
+        if (out) out[i] = float(v);
+        const uint32_t k = j % 3u;
+        acc[0] += k == 0 ? v : 0.0;
+        acc[1] += k == 1 ? v : 0.0;
+        acc[2] += k == 2 ? v : 0.0;
+    }
+    if (!steady) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off);
+    if (__lane_id() == 0)
+        for (int k = 0; k < 3; ++k) part[threadIdx.x >> 6][k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int q = 0; q < kNormBlock / 64; ++q) t += part[q][threadIdx.x];
+        steady[p * 3 + threadIdx.x] = float(t);
+    }
+}
+
 // batch closest hit for the public query entry point
 template <int TRAV, int STACK>
 __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, const tr_ray* __restrict__ rays,
@@ -1462,6 +1499,8 @@ struct tr_film_s {
     size_t n_elems = 0;  // max(layers, 1) * rows*W*T*3
     float *d_pass = nullptr, *d_sum = nullptr, *d_comp = nullptr;
     float *d_layer_w = nullptr, *d_steady = nullptr;  // multibounce films only
+    float *d_read = nullptr, *d_read_steady = nullptr;  // read-back staging (allocated by the first read)
+    double* d_read_w = nullptr;
     std::vector<double> weight;  // rows*W
     hipStream_t stream = nullptr;
 };
@@ -1900,6 +1939,7 @@ int tr_film_destroy(tr_film f)
     if (!f) return TR_OK;
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp); dev_free(f->d_layer_w); dev_free(f->d_steady);
+    dev_free(f->d_read); dev_free(f->d_read_steady); dev_free(f->d_read_w);
     if (f->stream) (void)hipStreamDestroy(f->stream);
     delete f;
     return TR_OK;
@@ -1950,13 +1990,31 @@ int tr_film_read(tr_film f, float* out, float* steady)
 {
     if (!f) return fail(TR_ERR_INVALID, "tr_film_read: null film");
     if (!out && !steady) return TR_OK;
+    const size_t W = size_t(f->desc.width), T = size_t(f->desc.time_bins), P = size_t(f->rows) * W;
+    if (!f->layers) {
+        // normalised on the device; one copy of the result instead of both raw arrays
+        int rc;
+        if (!f->d_read_w && (rc = dev_alloc(&f->d_read_w, P))) return rc;
+        if (out && !f->d_read && (rc = dev_alloc(&f->d_read, f->n_elems))) return rc;
+        if (steady && !f->d_read_steady && (rc = dev_alloc(&f->d_read_steady, P * 3))) return rc;
+        HIP_TRY(hipMemcpyAsync(f->d_read_w, f->weight.data(), P * sizeof(double), hipMemcpyHostToDevice, f->stream));
+        film_normalize_kernel<<<uint32_t(P), kNormBlock, 0, f->stream>>>(f->d_sum, f->d_comp, f->d_read_w,
+                                                                        uint32_t(3 * T), out ? f->d_read : nullptr,
+                                                                        steady ? f->d_read_steady : nullptr);
+        HIP_TRY(hipGetLastError());
+        if (out) HIP_TRY(hipMemcpyAsync(out, f->d_read, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+        if (steady)
+            HIP_TRY(hipMemcpyAsync(steady, f->d_read_steady, P * 3 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+        HIP_TRY(hipStreamSynchronize(f->stream));
+        return TR_OK;
+    }
     std::vector<float> s(f->n_elems), c(f->n_elems);
     HIP_TRY(hipMemcpyAsync(s.data(), f->d_sum, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
     HIP_TRY(hipMemcpyAsync(c.data(), f->d_comp, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
     HIP_TRY(hipStreamSynchronize(f->stream));
-    const size_t W = size_t(f->desc.width), T = size_t(f->desc.time_bins), P = size_t(f->rows) * W;
-    if (f->layers) {
-        // each bounce film divided by its own weights; the steady image holds every sample
+    {
+        // multibounce films: each bounce film divided by its own weights; the steady image holds
+        // every sample
         std::vector<float> lw(size_t(f->layers) * P), st(P * 3);
         HIP_TRY(hipMemcpyAsync(lw.data(), f->d_layer_w, lw.size() * sizeof(float), hipMemcpyDeviceToHost, f->stream));
         HIP_TRY(hipMemcpyAsync(st.data(), f->d_steady, st.size() * sizeof(float), hipMemcpyDeviceToHost, f->stream));
@@ -1979,21 +2037,6 @@ int tr_film_read(tr_film f, float* out, float* steady)
                 }
         return TR_OK;
     }
-    for (size_t p = 0; p < P; ++p) {
-        double w = f->weight[p];
-        double acc[3] = {0, 0, 0};
-        for (size_t t = 0; t < T; ++t)
-            for (int k = 0; k < 3; ++k) {
-                size_t i = (p * T + t) * 3 + size_t(k);
-                double v = (double(s[i]) + double(c[i])) / w;
-                if (!std::isfinite(v)) v = 0.0;  // 0/0 -> 0 (ImageRow::weight, Image.h:153-162)
-                if (out) out[i] = float(v);
-                acc[k] += v;
-            }
-        if (steady)
-            for (int k = 0; k < 3; ++k) steady[p * 3 + size_t(k)] = float(acc[k]);
-    }
-    return TR_OK;
 }
 
 int tr_film_write(tr_film f, const char* name, const char* ext)
