@@ -355,7 +355,9 @@ std::vector<Bvh4QNodeHost> quantize_bvh4(const Bvh4& b4)
             while (e < 254 && dequant(o, 255u, e) < h) ++e;
             be[a] = e;
         }
-        q.exps = uint32_t(be[0]) | uint32_t(be[1]) << 8 | uint32_t(be[2]) << 16;
+        q.step_x = std::ldexp(1.0f, be[0] - 127);
+        q.step_y = std::ldexp(1.0f, be[1] - 127);
+        q.step_z = std::ldexp(1.0f, be[2] - 127);
         // byte offsets: lo_x 0, hi_x 4, lo_y 8, hi_y 12 (word 1), lo_z 16, hi_z 20 (word 2)
         const int lo_off[3] = {0, 8, 16}, hi_off[3] = {4, 12, 20};
         for (int k = 0; k < 4; ++k) {

@@ -386,12 +386,11 @@ __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, co
         st.tfar = tfar;
         return done;
     }
-    // quantized node (Bvh4QNodeHost): origin, grid exponents, child box bytes, child references
+    // quantized node (Bvh4QNodeHost): origin, grid steps, child box bytes, child references
     const uint4* n = reinterpret_cast<const uint4*>(nodes) + 4 * size_t(st.node);
     const uint4 w0 = n[0], w1 = n[1], w2 = n[2], w3 = n[3];
     const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
-    const float sx = __uint_as_float((w0.w & 0xFFu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23),
-                sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
+    const float sx = __uint_as_float(w2.z), sy = __uint_as_float(w2.w), sz = __uint_as_float(w0.w);
     const uint32_t ref[4] = {w3.x, w3.y, w3.z, w3.w};
     int cha[4], cna[4];
 #pragma unroll

@@ -60,9 +60,10 @@ struct Bvh4 {
 Bvh4 collapse_bvh4(const BvhBuildResult& b);
 
 // ---- quantized 4-wide node: 64 bytes (four 16-byte words) --------------------------------------
-//   word 0: origin xyz (float), w = biased exponents ex | ey << 8 | ez << 16 (grid step 2^(e-127))
+//   word 0: origin xyz (float), w = grid step of z (a power of two, as a float)
 //   word 1: child box bytes lo_x[4], hi_x[4], lo_y[4], hi_y[4]
-//   word 2: lo_z[4], hi_z[4], 8 bytes unused
+//   word 2: lo_z[4], hi_z[4], grid steps of x and y (floats; stored whole, the traversal reads
+//           them without decoding)
 //   word 3: child references: index | count << 29 (count 0: inner node, 1..4: leaf triangles
 //           starting at index, 7: empty slot)
 // A child box is origin + q * step per plane, q a byte; lo planes round down and hi planes up, so
@@ -70,8 +71,9 @@ Bvh4 collapse_bvh4(const BvhBuildResult& b);
 // exact), and the traversal's slab test stays conservative.
 struct alignas(16) Bvh4QNodeHost {
     float origin[3];
-    uint32_t exps;
-    uint8_t q[32];
+    float step_z;
+    uint8_t q[24];
+    float step_x, step_y;
     uint32_t ref[4];
 };
 static_assert(sizeof(Bvh4QNodeHost) == 64, "quantized node layout");
