@@ -2,8 +2,8 @@
 """Throughput of the transient path tracer on one MI355X.
 
 A step is one full render pass of the workload (every path of every pixel,
-every bounce, the film accumulation and its compensated merge) with the scene
-and film already resident in HBM. Prints one JSON line.
+every bounce, and the compensated film accumulation) with the scene and film
+already resident in HBM. Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--workload icosphere6|box|media|batch64] [--guided on|only]
 
@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 BYTES_EXTEND_PER_RAY = 32 + 16          # ray in, hit out
 BYTES_SHADOW_PER_RAY = 32 + 1           # shadow ray in, visibility byte out
 BYTES_FILM_PER_RECORD = 1               # visibility byte of every shadow record
-BYTES_FILM_PER_CONTRIB = 20 + 24        # record in (RGB, pixel, bin) + FP32 RGB read-modify-write
+BYTES_FILM_PER_CONTRIB = 20 + 48        # record in (RGB, pixel, bin) + FP32 RGB sum and compensation RMW
 BYTES_SHADE_PER_SEGMENT = 64 + 64 + 20  # path state in + out, NEE record out
 BYTES_PATH_PER_SEGMENT = 300            # whole segment: extension + shadow + state + record + film
 
@@ -43,7 +43,7 @@ def workload(name: str):
                                  n_incoming=1, max_bounces=8, seed=0)
         desc = ("81,920-triangle displaced icosphere (subdivision 6, value-noise amplitude 0.05) in the open "
                 "unit box, 0.3x0.3 area light; 256x256 px x 1024 time bins of 0.01; 1024 spp "
-                "(32x32 camera samples x 1 path); max 8 bounces; FP32 film with compensated merge")
+                "(32x32 camera samples x 1 path); max 8 bounces; FP32 film with compensated per-bin sums")
     elif name == "box":
         s = scenes.cornell_box(width=64, height=64, time_bins=256, exposure=0.01, sqrt_spp=32, n_incoming=1,
                                max_bounces=4, seed=0)
@@ -74,18 +74,37 @@ def _cpu_worker(args):
     return st["paths"], st["render_seconds"]
 
 
-def cpu_baseline(name: str, seconds: float, max_procs: int = 16):
+def host_cores():
+    """(processes the CPU baseline may run, description of the host's cores). The baseline uses
+    every core this process may run on, up to the host's CPU share for one GPU when the
+    environment states one (OMP_NUM_THREADS; a shared GPU host gives each GPU's jobs that many)."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    procs = max(1, min(affinity, share) if share > 0 else affinity)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return procs, {"os_cpu_count": os.cpu_count(), "affinity": affinity, "cpu_share": share or None,
+                   "processes": procs, "model": model}
+
+
+def cpu_baseline(name: str, seconds: float):
     """The oracle on `procs` host cores for about `seconds`: each process renders a disjoint row
     range (or, for a scanline workload, the scanline with its own seed) of the same scene."""
     from oracle import pyoracle as po
     if not os.path.exists(po.LIB_PATH):
         po.build()
     s, _ = workload(name)
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    procs = max(1, min(cores, max_procs))
+    procs, host = host_cores()
     H, W, n_inc = s.film.height, s.film.width, s.render.n_incoming
     scan = s.film.scanline
     y_mid = scan if scan >= 0 else H // 2
@@ -108,7 +127,20 @@ def cpu_baseline(name: str, seconds: float, max_procs: int = 16):
               f"path(s) of the same scene (per-path RNG streams, as on the GPU): {paths} paths in {wall:.1f} s "
               f"wall; per core {per_core:.4g} paths/s of render loop")
     return {"value": paths / wall, "unit": "paths/s", "cores": procs, "kind": "port", "sample": sample,
-            "value_per_core": per_core}
+            "value_per_core": per_core, "host": host, "calibration": CPU_CALIBRATION}
+
+
+# The oracle's render loop against the reference binary (Embree BVH8 + AVX2, single-threaded), per
+# core on the same machine (an 8-core 2.7 GHz Xeon), best of three runs each, render time only.
+# Not calibrated on the 81,920-triangle scene: the reference builds only with Embree there (its
+# non-Embree intersection tests every triangle), and Embree's CMake build is out of scope here.
+CPU_CALIBRATION = {
+    "box (22 triangles, 64x64 px, 16 camera samples x 64 paths, 4 bounces)":
+        {"reference_paths_per_s": 1.89e6, "oracle_paths_per_s": 2.2e6, "ratio": 1.16},
+    "media scanline (22 triangles + medium, 512 px, 16 x 16 paths, 16 events)":
+        {"reference_paths_per_s": 0.63e6, "oracle_paths_per_s": 0.64e6, "ratio": 1.02},
+    "icosphere6 (81,930 triangles)": "not calibrated",
+}
 
 
 # ---------------------------------------------------------------------------------------------
@@ -280,17 +312,14 @@ def guided_comparison(name="icosphere_media", ref_sqrt=32, ref_spp=262144, pass_
 # ---------------------------------------------------------------------------------------------
 # Batched workload: 64 uneven scenes (seeds 0..63) rendered together by one launch sequence.
 
-def cpu_baseline_batch(n_jobs: int, seconds: float, max_procs: int = 16):
+def cpu_baseline_batch(n_jobs: int, seconds: float):
     """The oracle on host cores over a bounded sample of the batched workload: process p renders
-    the middle row of job p (of the first min(cores, 16) jobs) at a reduced sample count."""
+    the middle row of job p (of the first min(cores, n_jobs) jobs) at a reduced sample count."""
     from oracle import pyoracle as po
     if not os.path.exists(po.LIB_PATH):
         po.build()
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    procs = max(1, min(cores, max_procs, n_jobs))
+    cores, host = host_cores()
+    procs = max(1, min(cores, n_jobs))
     jobs = []
     for p in range(procs):
         s, _ = workload(f"batch64:{p}")
@@ -310,7 +339,7 @@ def cpu_baseline_batch(n_jobs: int, seconds: float, max_procs: int = 16):
               f"(per-path RNG streams, as on the GPU): {paths} paths in {wall:.1f} s wall; per core "
               f"{per_core:.4g} paths/s of render loop")
     return {"value": paths / wall, "unit": "paths/s", "cores": procs, "kind": "port", "sample": sample,
-            "value_per_core": per_core}
+            "value_per_core": per_core, "host": host, "calibration": CPU_CALIBRATION}
 
 
 def bench_batch(args) -> dict:
@@ -344,8 +373,7 @@ def bench_batch(args) -> dict:
            "shadow": BYTES_SHADOW_PER_RAY * prof["shadow_rays"],
            "shade": BYTES_SHADE_PER_SEGMENT * prof["extension_rays"],
            "film": BYTES_FILM_PER_RECORD * prof["shadow_rays"] + BYTES_FILM_PER_CONTRIB * prof["contributions"],
-           "camera": 48 * prof["paths"],
-           "film_merge": sum(2 * 4 * f.rows * f.width * f.time_bins * 3 for (_, _, _, f, _) in setups)}
+           "camera": 48 * prof["paths"]}
     dom = max(kern_ms, key=lambda n: kern_ms[n])
     avg_ms = kern_ms[dom] / max(kern_n[dom], 1)
     bpl = alg[dom] / max(kern_n[dom], 1)
@@ -525,6 +553,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--binary-bvh", action="store_true", help="traverse the binary BVH (default: 4-wide)")
+    ap.add_argument("--launch-flags", default="",
+                    help="extra TR_RENDER_* launch flags, comma-separated names (e.g. POSTPONE_LEAVES,STACK32)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration)")
@@ -570,8 +600,11 @@ def main():
     eng.wave_paths = args.wave
     # the timed steps carry HIP events around the dominant kernel (extend) only: an event pair
     # between two launches drains the GPU, and timing every kernel would add 2-14 % to a step
+    launch_flags = 0
+    for name in filter(None, args.launch_flags.split(",")):
+        launch_flags |= getattr(nat, "TR_RENDER_" + name.strip().upper())
     eng.profile = False
-    eng.extra_flags = nat.TR_RENDER_PROFILE_EXTEND
+    eng.extra_flags = nat.TR_RENDER_PROFILE_EXTEND | launch_flags
     eng.binary_bvh = args.binary_bvh
     for k in range(args.warmup):
         eng.render(None, world, integ, cam, film, sampler, seed=1000 + k)
@@ -588,7 +621,7 @@ def main():
     dom_n = sum(s["kernel_launches"][dom] for s in steps)
     # per-kernel split: the same steps again, every kernel timed (after the timed region)
     eng.profile = True
-    eng.extra_flags = 0
+    eng.extra_flags = launch_flags
     t1 = time.perf_counter()
     psteps = [eng.render(None, world, integ, cam, film, sampler, seed=k) for k in range(args.steps)]
     pwall = time.perf_counter() - t1
@@ -602,7 +635,6 @@ def main():
         "shadow": BYTES_SHADOW_PER_RAY * shd,
         "film": BYTES_FILM_PER_RECORD * shd + BYTES_FILM_PER_CONTRIB * con,
         "shade": BYTES_SHADE_PER_SEGMENT * ext,
-        "film_merge": 2 * 4 * film.rows * film.width * film.time_bins * 3 * args.steps,
         "camera": 48 * paths,  # direction, path id + stream state, light stream + pixel + free flight
     }
     avg_ms = dom_ms / max(dom_n, 1)
@@ -634,7 +666,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {desc}", "paths_per_step": paths // args.steps,
                    "triangles": scene.n_tris, "bvh": "binary" if args.binary_bvh else "4-wide",
-                   "parallelism": "single GPU"},
+                   "launch_flags": args.launch_flags or None, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": avg_ms, "launches": dom_n,

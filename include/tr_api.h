@@ -156,9 +156,13 @@ typedef struct {
     int32_t bounce_layers;
 } tr_film_desc;
 
-/* StreakCameraFilm::StreakCameraFilm (src/Film/StreakCameraFilm.h:91-121). Device buffers:
- * per-pass accumulation plus a Kahan/Neumaier-compensated FP32 sum, rows x W x T x 3 floats each
- * (rows = 1 in scanline mode). */
+/* StreakCameraFilm::StreakCameraFilm (src/Film/StreakCameraFilm.h:91-121). Device buffers: a
+ * compensated FP32 sum, two arrays of rows x W x T x 3 floats (rows = 1 in scanline mode): the
+ * running sum and the exact rounding errors of the additions into it. Every addition into an
+ * element (each sample of a render, each LDS partial the film kernel flushes, each caller sample)
+ * is compensated, so sum + comp is the exact total up to second-order terms (about n * 2^-48 of
+ * the total after n additions, against n * 2^-24 for a plain FP32 running sum). A third array of
+ * the same size is allocated by the first tr_film_read for the normalised copy. */
 int tr_film_create(tr_film* out, const tr_film_desc* desc);
 int tr_film_destroy(tr_film film);
 /* Film::clean: zero all sums and weights. */
@@ -170,11 +174,34 @@ int tr_film_clear(tr_film film);
  * camera samples that sent it samples, MultibounceStreakCameraFilm.h:137-157); steady holds every
  * sample, inside the time window or not (Film::add_samples, Film.h:263-283). */
 int tr_film_read(tr_film film, float* out, float* steady);
+/* tr_film_read for the rows [row_begin, row_begin + row_count) of the film only (row index 0 is
+ * the scanline in scanline mode): out[row][x][t][c] and steady[row][x][c] of those rows, as
+ * StreakCameraFilm::average_and_save_slice produces them one scanline at a time
+ * (StreakCameraFilm.h:357-397). Not for multibounce films. */
+int tr_film_read_rows(tr_film film, int32_t row_begin, int32_t row_count, float* out, float* steady);
 /* Un-normalised compensated sums (sum + compensation) as double, and per-pixel weights
  * ([layer][row][x] for multibounce films). */
 int tr_film_read_sums(tr_film film, double* sums, double* weights);
 /* Device pointers of the compensated-sum buffers (read-only use by the caller). */
 int tr_film_device_buffers(tr_film film, float** d_sum, float** d_comp);
+
+/* One time-resolved radiance sample for tr_film_add_samples (RadianceSample, RadianceSample.h:34-73,
+ * with the film pixel its camera sample lies in). */
+typedef struct {
+    int32_t x, y;     /* film pixel; y is the image row (the scanline itself in scanline mode) */
+    float time;       /* path length: the sample's time, already unwarped if the caller wants that */
+    float rgb[3];     /* radiance, already weighted by the camera sample's filter weight */
+    float weight;     /* added once to the pixel's weight (Film::m_weight): the camera sample's
+                       * weight on one of its samples, 0 on the others */
+    uint32_t pad0;
+} tr_film_sample; /* 32 bytes */
+
+/* StreakCameraFilm::add_samples (src/Film/StreakCameraFilm.h:139-192, 280-297) for samples the
+ * caller made: bin = floor((time - offset) / exposure), samples outside [0, T) are not deposited
+ * (their weight still counts), each deposit is a compensated addition by the film kernel that
+ * renders use. Samples outside the film are an error (nothing is added). Not for multibounce
+ * films. Host pointer; n <= 2^31. */
+int tr_film_add_samples(tr_film film, const tr_film_sample* samples, uint32_t n);
 /* StreakCameraFilm::write (src/Film/StreakCameraFilm.h:233-242, 357-437) and Film::write
  * (src/Film/Film.h:358-471) with Imaging::save (src/Image/ImageIO.h:147-194): one streak image
  * per scanline y, "<name>_%04d.<ext>" (T wide, W high; stored transposed when T < 8), ext "hdr"
@@ -235,10 +262,16 @@ enum {
      * kernels and the LDS-privatised film), one after another, and the smaller jobs together in one
      * shared sequence of wavefronts; without the flag every job shares one sequence. tr_autotune
      * tries both */
-    TR_RENDER_PER_JOB = 128u
+    TR_RENDER_PER_JOB = 128u,
+    /* closest-hit launches run node steps and triangle steps in separate wavefront-wide phases: a
+     * node's entered leaves wait until many lanes hold some (keyed 16-bit stack; BVHs of at most
+     * 65,536 nodes, otherwise ignored). Launch tuning; results do not change. */
+    TR_RENDER_POSTPONE_LEAVES = 512u
 };
 
 enum {
+    /* TR_K_FILM_MERGE: no longer launched (films accumulate compensated sums directly); kept so the
+     * statistics layout does not change */
     TR_K_CAMERA = 0, TR_K_EXTEND = 1, TR_K_SHADE = 2, TR_K_SHADOW = 3, TR_K_FILM_MERGE = 4, TR_K_FILM = 5,
     TR_K_GUIDE = 6, /* guided renders: training-record bookkeeping and Adam steps */
     TR_K_COUNT = 8
@@ -255,9 +288,10 @@ typedef struct {
 /* TransientRenderer::render (src/Renderer/TransientRenderer.h:77-201) driving
  * BidirectionalPathTracing::operator()(ray, samples, ...) in path-tracing-only SteadyState mode
  * (src/Integrator/BidirectionalPathTracing.h:1100-1140) and StreakCameraFilm::add_samples
- * (src/Film/StreakCameraFilm.h:139-192). Accumulates into `film` (successive calls are passes
- * merged with compensated summation). Random numbers: one PCG32 stream per camera sample,
- * per path and per path for light sampling, all seeded from desc->seed. */
+ * (src/Film/StreakCameraFilm.h:139-192). Accumulates into `film` with compensated additions
+ * (successive calls add further passes). Non-finite samples are zeroed (TransientRenderer.h:167-171).
+ * Random numbers: one PCG32 stream per camera sample, per path and per path for light sampling,
+ * all seeded from desc->seed. */
 int tr_render_transient(tr_scene scene, const tr_camera* camera, tr_film film,
                         const tr_render_desc* desc, tr_render_stats* stats);
 

@@ -28,6 +28,7 @@ TR_RENDER_TIME_SAMPLING = 16
 TR_RENDER_STACK32 = 32
 TR_RENDER_STACK_KEYED = 64
 TR_RENDER_PER_JOB = 128
+TR_RENDER_POSTPONE_LEAVES = 512
 K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FILM_MERGE, K_FILM, K_GUIDE = 0, 1, 2, 3, 4, 5, 6
 K_NAMES = {K_CAMERA: "camera", K_EXTEND: "extend", K_SHADE: "shade", K_SHADOW: "shadow",
            K_FILM_MERGE: "film_merge", K_FILM: "film", K_GUIDE: "guide"}
@@ -68,6 +69,16 @@ class FilmDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("time_bins", ctypes.c_int32),
                 ("exposure", ctypes.c_float), ("offset", ctypes.c_float), ("scanline", ctypes.c_int32),
                 ("camera_unwarp", ctypes.c_int32), ("bounce_layers", ctypes.c_int32)]
+
+
+class FilmSample(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int32), ("y", ctypes.c_int32), ("time", ctypes.c_float), ("rgb", ctypes.c_float * 3),
+                ("weight", ctypes.c_float), ("pad0", ctypes.c_uint32)]
+
+
+# numpy record layout of tr_film_sample (32 bytes)
+FILM_SAMPLE_DTYPE = np.dtype([("x", "<i4"), ("y", "<i4"), ("time", "<f4"), ("rgb", "<f4", (3,)),
+                              ("weight", "<f4"), ("pad0", "<u4")])
 
 
 class CameraDesc(ctypes.Structure):
@@ -146,9 +157,12 @@ SIGNATURES = {
     "tr_film_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "tr_film_clear": (ctypes.c_int, [ctypes.c_void_p]),
     "tr_film_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "tr_film_read_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
     "tr_film_read_sums": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "tr_film_device_buffers": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                               ctypes.POINTER(ctypes.c_void_p)]),
+    "tr_film_add_samples": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     "tr_film_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]),
     "tr_image_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "tr_render_transient": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CameraDesc), ctypes.c_void_p,

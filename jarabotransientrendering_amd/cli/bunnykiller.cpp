@@ -35,7 +35,7 @@ struct MeshArg {
     std::string file;
     tr::Vector3 kd;
     bool two_sided;
-    bool default_material;
+    bool default_material;  // no material flag: the grey default, whose albedo is the last -lambertian-rho
 };
 
 struct Params {
@@ -56,6 +56,7 @@ struct Params {
     bool has_medium = false;
     tr::Vector3 sa{}, ss{};
     bool time_sampling = false;
+    bool print_parsed = false;  // --print-parsed: print what the command line set up, render nothing
 };
 
 [[noreturn]] void unsupported(const std::string& flag)
@@ -172,14 +173,17 @@ void parse(Args& A, Params& p)
             }
         } else if (f == "-max-nb-bounces") p.max_bounces = A.n();
         else if (f == "-bidirectional-path-tracing") p.n_incoming = unsigned(A.n());
-        else if (f == "-new-seed") p.seed = uint64_t(std::atoll(A.next().c_str()));
+        else if (f == "-new-seed") p.seed = uint64_t(std::atoi(A.next().c_str()));  // atoi, as the reference
         else if (f == "-log-name") p.log_file = A.next();
         else if (f == "-transient-state") {
         } else if (f == "-standard-sampling") p.time_sampling = false;
         else if (f == "-time-sampling") p.time_sampling = true;
         else if (f == "-scattering-level") {
-            if (A.next() != "all") unsupported("-scattering-level other than all");
+            // multiple / single / none select other components; anything else is ALL (bunnykiller.cpp:1372-1385)
+            const std::string lv = A.next();
+            if (lv == "multiple" || lv == "single" || lv == "none") unsupported("-scattering-level " + lv);
         } else if (f == "-parse-args-file") parse_file(A.next(), p);
+        else if (f == "--print-parsed") p.print_parsed = true;
         else if (f == "-steady-state" || f == "-photon-mapping" || f == "-sphere" ||
                  f == "-plane" || f == "-mie-medium" || f == "-cosine-light-source" ||
                  f == "-hemispherical-light-source" || f == "-point-light-source-polarized" ||
@@ -193,19 +197,99 @@ void parse(Args& A, Params& p)
     }
 }
 
+void vec(const char* key, const tr::Vector3& v, bool comma = true)
+{
+    std::printf("\"%s\": [%.9g, %.9g, %.9g]%s", key, double(v[0]), double(v[1]), double(v[2]), comma ? ", " : "");
+}
+
+// What the command line set up, as JSON: the film, camera, sampler and integrator settings, the lights
+// (with main's default light), the medium and every mesh with its material and triangle count.
+int print_parsed(const Params& p)
+{
+    std::printf("{\"film\": {\"width\": %d, \"height\": %d, \"time_bins\": %d, \"exposure\": %.9g, "
+                "\"offset\": %.9g, \"scanline\": %d, \"aspect_ratio\": 1, \"camera_unwarp\": %d, "
+                "\"bounce_layers\": %d, \"transient\": 1, \"image_file\": \"%s\", \"log_file\": \"%s\"}, ",
+                p.width, p.height, p.time, double(p.exposure), double(p.offset), p.scanline, int(p.unwarp), p.multibounce,
+                p.image_file.c_str(), p.log_file.c_str());
+    std::printf("\"camera\": {");
+    vec("position", p.cam_pos);
+    vec("focus", p.cam_focus);
+    vec("up", p.cam_up);
+    std::printf("\"view_plane_dist\": %.9g}, ", double(p.view_plane));
+    std::printf("\"sampler\": {\"sqrt_spp\": %d}, ", p.sqrt_spp);
+    std::printf("\"integrator\": {\"n_incoming\": %u, \"max_bounces\": %d, \"time_sampling\": %d, "
+                "\"path_tracing\": 1, \"seed\": %llu, \"scattering_level_all\": 1}, ",
+                p.n_incoming, p.max_bounces, int(p.time_sampling), (unsigned long long)p.seed);
+    std::printf("\"lights\": [");
+    bool first = true;
+    for (const auto& l : p.points) {
+        std::printf("%s{\"type\": \"point\", ", first ? "" : ", ");
+        vec("position", l.position);
+        vec("intensity", l.intensities, false);
+        std::printf("}");
+        first = false;
+    }
+    for (const auto& l : p.rects) {
+        std::printf("%s{\"type\": \"rect\", ", first ? "" : ", ");
+        vec("position", l.position);
+        vec("target", l.target);
+        std::printf("\"size\": [%.9g, %.9g], ", double(l.size[0]), double(l.size[1]));
+        vec("intensity", {l.intensity, l.intensity, l.intensity}, false);
+        std::printf("}");
+        first = false;
+    }
+    std::printf("], \"medium\": ");
+    if (p.has_medium) {
+        std::printf("{");
+        vec("sigma_a", p.sa);
+        vec("sigma_s", p.ss, false);
+        std::printf("}");
+    } else {
+        std::printf("null");
+    }
+    std::printf(", \"meshes\": [");
+    for (size_t k = 0; k < p.meshes.size(); ++k) {
+        const MeshArg& m = p.meshes[k];
+        float *xyz = nullptr, *nrm = nullptr;
+        uint32_t *idx = nullptr, nv = 0, nt = 0;
+        if (tr_obj_load(m.file.c_str(), &xyz, &nv, &idx, &nt, &nrm) != TR_OK) {
+            std::fprintf(stderr, "%s\n", tr_last_error());
+            return 1;
+        }
+        tr_obj_free(xyz, idx, nrm);
+        std::printf("%s{\"file\": \"%s\", \"triangles\": %u, \"two_sided\": %d, \"default_material\": %d, ",
+                    k ? ", " : "", m.file.c_str(), nt, int(m.two_sided), int(m.default_material));
+        vec("kd", m.kd, false);
+        std::printf("}");
+    }
+    std::printf("]}\n");
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
 {
     try {
-        for (int i = 0; i < argc; i++) std::fprintf(stdout, "argv[%d]: %s\n", i, argv[i]);
-        std::fprintf(stdout, "\n");
         Params p;
         Args A;
         A.a.assign(argv, argv + argc);
         A.i = 1;
+        bool quiet = false;
+        for (const auto& a : A.a) quiet = quiet || a == "--print-parsed";
+        if (!quiet) {
+            for (int i = 0; i < argc; i++) std::fprintf(stdout, "argv[%d]: %s\n", i, argv[i]);
+            std::fprintf(stdout, "\n");
+        }
         parse(A, p);
         if (p.log_file.empty()) p.log_file = p.image_file + ".log";
+        // meshes without a material flag take the grey default, set up after all arguments are read
+        // (bunnykiller.cpp:1499-1501), so the last -lambertian-rho applies to all of them
+        for (auto& m : p.meshes)
+            if (m.default_material) m.kd = {p.grey, p.grey, p.grey};
+        // no light on the command line: a unit point light at the camera (bunnykiller.cpp:1636-1645)
+        if (p.points.empty() && p.rects.empty()) p.points.emplace_back(p.cam_pos, tr::Vector3{1.f, 1.f, 1.f});
+        if (p.print_parsed) return print_parsed(p);
 
         tr::World w;
         for (const auto& m : p.meshes) w.add_triangle_mesh(m.file, tr::Lambertian(m.kd), m.two_sided);

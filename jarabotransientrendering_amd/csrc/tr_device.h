@@ -287,13 +287,14 @@ constexpr uint32_t kQEmptyDev = 7u;  // child reference count of an empty slot (
 
 // Per-lane state of a 4-wide BVH query between node steps.
 struct Trav4 {
-    int32_t node;      // node to visit next; while leaves are tested, the node they belong to
+    int32_t node;      // node to visit next (-1: none); while leaves are tested, the node they belong to
     int32_t sp;        // stack entries
     int32_t hit;       // closest (or any) hit so far, -1 if none
     float tfar;        // current far bound (the hit distance once there is a hit)
     uint32_t tri;      // leaf mode (tri < tri_end): next triangle of the current leaf
     uint32_t tri_end;  // end of the current leaf's triangles
     uint32_t leaves;   // leaf mode: entered leaf children of `node` not finished (the lowest bit is current)
+    int32_t pnode;     // postponed leaves (closest hit): the node whose leaves `leaves` are
 };
 
 __device__ __forceinline__ void trav4_begin(Trav4& st, float tfar)
@@ -305,6 +306,7 @@ __device__ __forceinline__ void trav4_begin(Trav4& st, float tfar)
     st.tri = 0;
     st.tri_end = 0;
     st.leaves = 0;
+    st.pnode = -1;
 }
 
 // One step of a closest (ANY = false) or any (ANY = true) hit query over the quantized 4-wide BVH
@@ -496,6 +498,131 @@ __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, co
     st.tfar = tfar;
     return done;
 }
+// ---- closest hit with postponed leaves -------------------------------------------------------
+// A closest-hit query whose wavefront runs node steps and triangle steps as separate, wavefront-wide
+// phases: a node step records the node's entered leaves as the lane's pending leaves instead of
+// testing them, and the lane goes on to the nearest inner child (or pops the keyed stack) while
+// its leaves wait; the caller runs a triangle phase (one triangle per lane with pending leaves)
+// once enough lanes hold some, or when no lane can take a node step. A lane that meets a second
+// node with entered leaves while it still holds pending ones waits on that node and visits it
+// again after its leaves are done (with the smaller far bound). Nodes and triangles then run with
+// most lanes active instead of every node step carrying the triangle loop of a few lanes.
+// Needs the keyed stack (kStackKeyed): entries behind a hit found later are dropped on pop.
+
+// true when the lane can take a node step now
+__device__ __forceinline__ bool trav4_pp_can_step(const Trav4& st, bool blocked) { return st.node >= 0 && !blocked; }
+
+// One node step; `blocked` is set when the node has entered leaves while the lane holds pending
+// ones (the state is left unchanged: the node is visited again once they are tested). Returns true
+// when the query is complete.
+template <int SK>
+__device__ __forceinline__ bool trace4_node_pp(const float4* __restrict__ nodes, const RayPre& r, float tnear, Trav4& st,
+                                               uint32_t* stack, int tid, int stride, bool& blocked)
+{
+    static_assert(SK == kStackKeyed || SK == kStack32, "postponed leaves use the keyed (or 32-bit) stack");
+    const float tfar = st.tfar;
+    int sp = st.sp;
+    const uint4* n = reinterpret_cast<const uint4*>(nodes) + 4 * size_t(st.node);
+    const uint4 w0 = n[0], w1 = n[1], w2 = n[2], w3 = n[3];
+    const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+    const float sx = __uint_as_float(w2.z), sy = __uint_as_float(w2.w), sz = __uint_as_float(w0.w);
+    const uint32_t ref[4] = {w3.x, w3.y, w3.z, w3.w};
+    int cha[4], cna[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        cha[k] = int(ref[k] & 0x1FFFFFFFu);
+        const int c = int(ref[k] >> 29);
+        cna[k] = c == int(kQEmptyDev) ? -1 : c;
+    }
+    // the slab test of trace4_step (same planes, same error bound)
+    const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
+    const float bx = (ox - r.o.x) * r.inv.x, by = (oy - r.o.y) * r.inv.y, bz = (oz - r.o.z) * r.inv.z;
+    const float ex = 3e-7f * fmaf(fmaf(255.0f, sx, fabsf(ox)), fabsf(r.inv.x), fabsf(bx));
+    const float ey = 3e-7f * fmaf(fmaf(255.0f, sy, fabsf(oy)), fabsf(r.inv.y), fabsf(by));
+    const float ez = 3e-7f * fmaf(fmaf(255.0f, sz, fabsf(oz)), fabsf(r.inv.z), fabsf(bz));
+    const float bx0 = bx - ex, bx1 = bx + ex, by0 = by - ey, by1 = by + ey, bz0 = bz - ez, bz1 = bz + ez;
+    const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
+    const uint32_t nx = px ? w1.x : w1.y, fx = px ? w1.y : w1.x;
+    const uint32_t ny = py ? w1.z : w1.w, fy = py ? w1.w : w1.z;
+    const uint32_t nz = pz ? w2.x : w2.y, fz = pz ? w2.y : w2.x;
+    float key[4];
+    int idx[4];
+    uint32_t leaves = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float x0 = fmaf(float((nx >> (8 * k)) & 0xFFu), ax, bx0), x1 = fmaf(float((fx >> (8 * k)) & 0xFFu), ax, bx1);
+        const float y0 = fmaf(float((ny >> (8 * k)) & 0xFFu), ay, by0), y1 = fmaf(float((fy >> (8 * k)) & 0xFFu), ay, by1);
+        const float z0 = fmaf(float((nz >> (8 * k)) & 0xFFu), az, bz0), z1 = fmaf(float((fz >> (8 * k)) & 0xFFu), az, bz1);
+        const float t0 = fmaxf(fmaxf(x0, y0), fmaxf(z0, tnear));
+        const float t1 = fminf(fminf(x1, y1), fminf(z1, tfar)) * 1.00000036f;
+        bool h = t0 <= t1 && cna[k] >= 0;
+        if (h && cna[k] > 0) leaves |= 1u << k;
+        h = h && cna[k] == 0;
+        key[k] = h ? t0 : __int_as_float(0x7f800000);
+        idx[k] = h ? cha[k] : -1;
+    }
+    if (leaves && st.leaves) {
+        blocked = true;  // test the pending leaves first, then visit this node again
+        return false;
+    }
+    if (leaves) {
+        const int k = __builtin_ctz(leaves);
+        st.pnode = st.node;
+        st.leaves = leaves;
+        st.tri = uint32_t(sel4(cha[0], cha[1], cha[2], cha[3], k));
+        st.tri_end = st.tri + uint32_t(sel4(cna[0], cna[1], cna[2], cna[3], k));
+    }
+#define TR_CSWAP(a, b)                                                            \
+    if (key[b] < key[a]) {                                                        \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk;                          \
+        int ti = idx[a]; idx[a] = idx[b]; idx[b] = ti;                            \
+    }
+    TR_CSWAP(0, 1) TR_CSWAP(2, 3) TR_CSWAP(0, 2) TR_CSWAP(1, 3) TR_CSWAP(1, 2)
+#undef TR_CSWAP
+#pragma unroll
+    for (int k = 3; k >= 1; --k)
+        if (idx[k] >= 0) {
+            if constexpr (SK == kStack32) stack[sp * stride + tid] = uint32_t(idx[k]);
+            else stack[sp * stride + tid] = (__float_as_uint(key[k]) & 0xFFFF0000u) | uint32_t(idx[k]);
+            ++sp;
+        }
+    int32_t next = idx[0];
+    while (next < 0 && sp > 0) {
+        --sp;
+        const uint32_t e = stack[sp * stride + tid];
+        if constexpr (SK == kStack32) {
+            next = int32_t(e);
+        } else {
+            if (__uint_as_float(e & 0xFFFF0000u) > tfar * 1.00000036f) continue;
+            next = int32_t(e & 0xFFFFu);
+        }
+    }
+    st.node = next;
+    st.sp = sp;
+    return next < 0 && st.leaves == 0;
+}
+
+// One triangle of the lane's pending leaves. Returns true when the query is complete.
+__device__ __forceinline__ bool trace4_leaf_pp(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                               const RayPre& r, float tnear, Trav4& st, bool& blocked)
+{
+    tri_hit(r, reinterpret_cast<const float*>(tris + 4 * size_t(st.tri)), tnear, st.tfar, st.hit);
+    if (++st.tri == st.tri_end) {
+        const uint32_t rest = st.leaves & (st.leaves - 1u);
+        st.leaves = rest;
+        if (rest) {
+            const uint4 w3 = reinterpret_cast<const uint4*>(nodes)[4 * size_t(st.pnode) + 3];
+            const int k = __builtin_ctz(rest);
+            const uint32_t ref = uint32_t(sel4(int32_t(w3.x), int32_t(w3.y), int32_t(w3.z), int32_t(w3.w), k));
+            st.tri = ref & 0x1FFFFFFFu;
+            st.tri_end = st.tri + (ref >> 29);
+        } else {
+            blocked = false;
+        }
+    }
+    return st.leaves == 0 && st.node < 0;
+}
+
 template <bool ANY, int SK>
 __device__ __forceinline__ int32_t trace4(const float4* __restrict__ nodes, const float4* __restrict__ tris,
                                           const RayPre& r, float tnear, float& tfar, uint32_t* stack, int tid,

@@ -31,6 +31,13 @@ using namespace trg;
 
 namespace {
 
+// waits for a stream's queued work when it goes out of scope (asynchronous uploads from host
+// buffers that live on the caller's stack)
+struct StreamDrain {
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
+
 constexpr int kRecFloats = 16;  // training record: 4 float4 (see GuideView / record layout below)
 // record layout: [0] pos (normalised) xyz, t/total_time; [1] wo xyz, medium flag;
 //                [2] normal xyz, weight (F/p before normalisation); [3] sampled direction xyz, 0
@@ -743,6 +750,8 @@ int tr_guide_gradient(tr_guide g, const float* records, uint32_t n, float* grad,
     for (uint32_t i = 0; i < n; ++i) ws += records[size_t(i) * kRecFloats + 11];
     uint32_t cnt[4] = {n, 0, 0, 0};
     float wsv[4] = {float(ws), 0.f, 0.f, 0.f};
+    // the uploads read cnt / wsv / records asynchronously: every return waits for them first
+    StreamDrain drain{g->stream};
     GHIP_TRY(hipMemcpyAsync(d_rec, records, size_t(n) * kRecFloats * sizeof(float), hipMemcpyHostToDevice, g->stream));
     GHIP_TRY(hipMemcpyAsync(g->count, cnt, sizeof cnt, hipMemcpyHostToDevice, g->stream));
     GHIP_TRY(hipMemcpyAsync(g->wsum, wsv, sizeof wsv, hipMemcpyHostToDevice, g->stream));
@@ -791,6 +800,7 @@ int tr_guide_train_batch(tr_guide g, const float* records, uint32_t n, float* lo
     for (uint32_t i = 0; i < n; ++i) ws += records[size_t(i) * kRecFloats + 11];
     uint32_t cnt[4] = {n, 0, 0, 0};
     float wsv[4] = {float(ws), 0.f, 0.f, 0.f};
+    StreamDrain drain{g->stream};
     GHIP_TRY(hipMemcpyAsync(g->batch, records, size_t(n) * kRecFloats * sizeof(float), hipMemcpyHostToDevice, g->stream));
     GHIP_TRY(hipMemcpyAsync(g->count, cnt, sizeof cnt, hipMemcpyHostToDevice, g->stream));
     GHIP_TRY(hipMemcpyAsync(g->wsum, wsv, sizeof wsv, hipMemcpyHostToDevice, g->stream));

@@ -2,11 +2,12 @@
 // reference writes through Imaging::save (src/Image/ImageIO.h:147-194).
 //
 // RGBE follows Greg Ward's Radiance picture format ("Real Pixels", Graphics Gems II, 1991): a
-// text header, a resolution line "-Y <height> +X <width>", then scanlines of 4-byte pixels
-// (mantissas r, g, b scaled by a shared exponent byte e: value = m * 2^(e - 136)). Scanlines of
-// 8..32767 pixels use the run-length encoding of the format's "new" scheme: a 4-byte marker
-// (2, 2, width >> 8, width & 255), then each of the four byte planes as runs (count > 128:
-// count - 128 repeats of one byte) and literal spans (count <= 128 bytes).
+// text header, a resolution line "-Y <height> +X <width>", then 4-byte pixels (mantissas r, g, b
+// scaled by a shared exponent byte e: value = m * 2^(e - 136)). The files carry the same bytes as
+// the reference's: its writer (the vendored stb_image_write, stb_image_write.h:588-640, 687-705)
+// emits the header lines "#?RADIANCE", "FORMAT=32-bit_rle_rgbe", "EXPOSURE=" with a fixed value,
+// and every scanline as flat pixels (its run-length branch is disabled), so a reader of the
+// reference's output reads these files alike.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -36,65 +37,20 @@ void rgbe_from_float(float r, float g, float b, uint8_t out[4])
     out[3] = uint8_t(e + 128);
 }
 
-namespace {
-
-// one byte plane of a scanline in the run-length scheme
-void encode_plane(const uint8_t* v, int n, std::vector<uint8_t>& out)
-{
-    int x = 0;
-    while (x < n) {
-        // find the next run of at least 4 equal bytes starting at or after x
-        int run_start = x, run_len = 0;
-        while (run_start < n) {
-            run_len = 1;
-            while (run_start + run_len < n && run_len < 127 && v[run_start + run_len] == v[run_start]) ++run_len;
-            if (run_len >= 4) break;
-            run_start += run_len;
-            run_len = 0;
-        }
-        // literal bytes before the run, in spans of at most 128
-        while (x < run_start) {
-            int k = run_start - x;
-            if (k > 128) k = 128;
-            out.push_back(uint8_t(k));
-            out.insert(out.end(), v + x, v + x + k);
-            x += k;
-        }
-        if (run_len >= 4) {
-            out.push_back(uint8_t(128 + run_len));
-            out.push_back(v[run_start]);
-            x = run_start + run_len;
-        }
-    }
-}
-
-}  // namespace
-
 int write_rgbe(const char* path, const float* rgb, int width, int height)
 {
     FILE* f = std::fopen(path, "wb");
     if (!f) return fail(TR_ERR_INVALID, std::string("Error: failed to open output file: \"") + path + "\"");
-    std::string head = "#?RADIANCE\n# transient path tracer\nFORMAT=32-bit_rle_rgbe\n\n-Y " + std::to_string(height) +
-                       " +X " + std::to_string(width) + "\n";
+    char res[128];
+    const int n = std::snprintf(res, sizeof res, "-Y %d +X %d\n", height, width);
+    const std::string head = std::string("#?RADIANCE\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=          1.0000000000000\n\n") +
+                             std::string(res, size_t(n));
     bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
-    std::vector<uint8_t> px(size_t(width) * 4), planes(size_t(width) * 4), out;
+    std::vector<uint8_t> px(size_t(width) * 4);
     for (int y = 0; y < height && ok; ++y) {
         const float* row = rgb + size_t(y) * size_t(width) * 3;
         for (int x = 0; x < width; ++x) rgbe_from_float(row[3 * x], row[3 * x + 1], row[3 * x + 2], &px[4 * size_t(x)]);
-        out.clear();
-        if (width < 8 || width > 32767) {
-            out.assign(px.begin(), px.end());
-        } else {
-            out.push_back(2);
-            out.push_back(2);
-            out.push_back(uint8_t(width >> 8));
-            out.push_back(uint8_t(width & 255));
-            for (int c = 0; c < 4; ++c) {
-                for (int x = 0; x < width; ++x) planes[size_t(c) * width + x] = px[4 * size_t(x) + c];
-                encode_plane(&planes[size_t(c) * width], width, out);
-            }
-        }
-        ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+        ok = std::fwrite(px.data(), 1, px.size(), f) == px.size();
     }
     ok = (std::fclose(f) == 0) && ok;
     if (!ok) return fail(TR_ERR_INVALID, std::string("Error: failed to write to output file: ") + path);

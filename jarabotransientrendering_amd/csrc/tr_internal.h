@@ -5,6 +5,22 @@
 #include <string>
 #include <vector>
 
+// ---- multibounce wavefronts ------------------------------------------------------------------
+// A multibounce film's shadow record tags its camera sample within the wavefront and its layer in
+// 32 bits: (camera sample) << kMbLayerTagBits | layer. A wavefront therefore holds whole camera
+// samples (n_inc paths each), at most kMbMaxCameraSamples - 1 of them.
+constexpr uint32_t kMbLayerTagBits = 5;  // layers <= 32
+constexpr uint32_t kMbMaxCameraSamples = 1u << (32 - kMbLayerTagBits);
+
+// wavefront size for a multibounce render asked to run `wave` paths per wavefront
+inline uint32_t mb_wave(uint32_t wave, uint32_t n_inc)
+{
+    const uint64_t cap = uint64_t(kMbMaxCameraSamples - 1) * n_inc;
+    const uint64_t w = wave < cap ? wave : cap;
+    const uint64_t whole = w / n_inc * n_inc;
+    return uint32_t(whole > n_inc ? whole : n_inc);
+}
+
 // ---- BVH (binary, both child boxes stored in the parent; 64 bytes per node) ----------------
 //   a = (L.lo.x, L.hi.x, L.lo.y, L.hi.y)
 //   b = (R.lo.x, R.hi.x, R.lo.y, R.hi.y)

@@ -9,11 +9,9 @@
 //     shade_kernel                free-flight correction, vertex, NEE light
 //                                 sample -> shadow queue, roulette, next
 //                                 direction -> next ray queue               (BidirectionalPathTracing.h:486-692)
-//     shadow_kernel               occlusion of every shadow ray; visible
-//                                 contributions are binned into the pass
-//                                 film with float atomics                   (World::intersects,
-//                                                                            StreakCameraFilm.h:139-192)
-//   film_merge_kernel             pass film -> Neumaier-compensated FP32 film (once per render call)
+//     shadow_kernel               occlusion of every shadow ray             (World::intersects)
+//     film_kernel                 visible contributions binned into the
+//                                 film's compensated FP32 (sum, comp) pairs (StreakCameraFilm.h:139-192)
 //
 // Path state lives in HBM as structure-of-arrays float4/uint4 records in two buffers: the rays of
 // bounce k are records [0, count) of buffer k & 1, and shade writes the paths that continue,
@@ -67,6 +65,8 @@ constexpr float kSigmaVisibility = 1e-5f;             // _SIGMA_VISIBILITY_ (bun
 constexpr int32_t kMatTwoSided = 1 << 30;
 constexpr int32_t kMatSmooth = 1 << 29;
 constexpr int32_t kMatIndex = kMatSmooth - 1;
+// multibounce films: bits of a shadow record's tag that hold its layer (at most 32 layers)
+constexpr uint32_t kMaxLayersTagBits = kMbLayerTagBits;
 
 // counters[] layout (uint32 words)
 enum {
@@ -93,7 +93,8 @@ struct SceneDev {
 
 struct RenderDev {
     // film
-    float* film;               // pass buffer, [rows][W][T][3]
+    float* film;               // compensated sums [rows][W][T][3]: sum ...
+    float* film_comp;          // ... and the rounding errors of its additions (film_add)
     int32_t W, H, T, row0, spp, spp2, n_inc;
     float exposure, offset, total_time, inv_inc;
     int32_t unwarp, max_bounces;
@@ -104,7 +105,8 @@ struct RenderDev {
     // MultibounceStreakFilm: `layers` bounce films of layer_stride bins each (0 = plain film)
     int32_t layers;
     uint32_t layer_stride;
-    float* steady;             // [rows][W][3] every sample, any time (multibounce films)
+    float* steady;             // [rows][W][3] every sample, any time (multibounce films) ...
+    float* steady_comp;        // ... and its compensation
     float* layer_w;            // [layers][rows][W] camera samples with samples of each layer
     uint32_t* cs_bits;         // per camera sample of the wavefront: layers that received samples
     // Transient sampling mode (-time-sampling): free-flight rate 4 / (T * exposure); 0 = steady state
@@ -248,7 +250,11 @@ __global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S0, RenderDev R
 // ---------------------------------------------------------------------------------------------
 // Traversal variants: the binary BVH, or the 4-wide BVH with 32-bit, 16-bit or keyed 16-bit stack
 // entries (trd::kStack*). Any-hit queries of a keyed launch use the 16-bit format.
-enum { kTravBinary = 0, kTravWide = 1, kTravWide16 = 2, kTravWideKeyed = 3 };
+// kTravWidePP: keyed stack and closest hits with postponed leaves (trace4_node_pp / trace4_leaf_pp);
+// its any-hit launches use the 16-bit stack.
+enum { kTravBinary = 0, kTravWide = 1, kTravWide16 = 2, kTravWideKeyed = 3, kTravWidePP = 4 };
+// postponed leaves: a triangle phase runs once this many lanes of a wavefront hold pending leaves
+constexpr int kLeafPhaseMin = 32;
 
 // LDS words per thread of a traversal block for a stack of STACK entries
 template <int TRAV, int STACK>
@@ -264,6 +270,8 @@ __device__ __forceinline__ int32_t trace_scene(const SceneDev& S, const RayPre& 
         return trace4<ANY, kStack32>(S.nodes4, S.tris, r, tnear, tfar, stack, threadIdx.x, kTraceBlock);
     else if constexpr (TRAV == kTravWide16 || ANY)
         return trace4<ANY, kStack16>(S.nodes4, S.tris, r, tnear, tfar, stack, threadIdx.x, kTraceBlock);
+    else if constexpr (TRAV == kTravWidePP)
+        return trace4<ANY, kStackKeyed>(S.nodes4, S.tris, r, tnear, tfar, stack, threadIdx.x, kTraceBlock);
     else
         return trace4<ANY, kStackKeyed>(S.nodes4, S.tris, r, tnear, tfar, stack, threadIdx.x, kTraceBlock);
 }
@@ -350,7 +358,47 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
         atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_EXT), (unsigned long long)count);
     }
     const uint32_t chunk = fetch_chunk(count);
-    if constexpr (TRAV != kTravBinary) {
+    if constexpr (TRAV == kTravWidePP) {
+        // persistent grid with per-lane refill; node steps and triangle steps in wavefront-wide phases
+        RayRefill rf(count, chunk);
+        bool busy = false, blocked = false;
+        uint32_t rec = 0;
+        RayPre r;
+        Trav4 st;
+        const float4 *nodes = S0.nodes4, *tris = S0.tris;
+        for (;;) {
+            rf.run(count, chunk, Wv.counters + C_FETCH_EXT, busy, [&](uint32_t i) {
+                rec = i;
+                const float4 d = Wv.ray_d[cur][i];
+                float3 c = Wv.cpos;
+                if constexpr (BATCH) {
+                    const JobDev& J = jobs[__float_as_uint(d.w)];
+                    nodes = J.S.nodes4;
+                    tris = J.S.tris;
+                    c = J.R.cpos;
+                }
+                const float4 o = first ? make_float4(c.x, c.y, c.z, 0.0f) : Wv.ray_o[cur][i];
+                r = ray_pre(mk(o), mk(d));
+                trav4_begin(st, __int_as_float(0x7f800000));
+                blocked = false;
+            });
+            if (__ballot(busy) == 0ull) break;  // the queue is drained and every query complete
+            const bool can = busy && trav4_pp_can_step(st, blocked);
+            const bool pend = busy && st.leaves != 0u;
+            const uint64_t cm = __ballot(can), pm = __ballot(pend);
+            bool done = busy && !can && !pend;  // (every busy lane can step or holds leaves)
+            if (cm == 0ull || __popcll(pm) >= kLeafPhaseMin) {
+                if (pend) done = trace4_leaf_pp(nodes, tris, r, kSigmaVisibility, st, blocked);
+            } else if (can) {
+                done = trace4_node_pp<kStackKeyed>(nodes, r, kSigmaVisibility, st, stack, threadIdx.x, kTraceBlock,
+                                                   blocked);
+            }
+            if (busy && done) {
+                Wv.hit[rec] = make_float2(st.hit >= 0 ? st.tfar : -1.0f, __int_as_float(st.hit));
+                busy = false;
+            }
+        }
+    } else if constexpr (TRAV != kTravBinary) {
         // persistent grid with per-lane refill: every lane steps its own query one node at a time
         RayRefill rf(count, chunk);
         bool busy = false;
@@ -739,7 +787,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                             sh_o = make_float4(lp.x, lp.y, lp.z, ldist);
                             sh_d = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(fidx));
                             sh_c = make_float4(val.x, val.y, val.z, __uint_as_float(extra.z >> 8));
-                            sh_tag = ((slot / uint32_t(R.n_inc)) << 5) | layer;
+                            sh_tag = ((slot / uint32_t(R.n_inc)) << kMaxLayersTagBits) | layer;
                         } else if (in_window && finite && nonzero) {
                             uint32_t bin = min(uint32_t(floorf(sensor)), uint32_t(R.T - 1));
                             uint32_t fidx = (extra.z >> 8) * uint32_t(R.T) + bin;
@@ -977,28 +1025,70 @@ __global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, con
 }
 
 // ---------------------------------------------------------------------------------------------
-// film: bins the visible contributions of this bounce into the pass film. A block takes a range
-// of kFilmRange consecutive shadow records; their pixels are few (paths are enumerated pixel-major
+// film accumulation with compensated sums. Every film element is a pair of FP32 accumulators
+// (sum, comp). An addition of x is an atomic add that returns the old value a; the rounding error
+// of the stored a + x is then recovered exactly with Knuth's TwoSum and added to comp, so after
+// any number of additions in any order sum + comp equals the exact total up to the rounding of the
+// additions into comp (errors of the errors: second order, about n * eps^2 of the total instead of
+// the n * eps of a plain FP32 running sum). The hardware atomic rounds to nearest-even like the
+// vector ALU, which is what makes the recovered error exact.
+__device__ __forceinline__ float two_sum_err(float a, float b, float s)
+{
+    // s = fl(a + b); returns (a + b) - s exactly
+    const float bb = s - a;
+    return (a - (s - bb)) + (b - bb);
+}
+
+// x into the compensated pair (sum[i], comp[i]) held in global memory
+__device__ __forceinline__ void film_add(float* __restrict__ sum, float* __restrict__ comp, size_t i, float x)
+{
+    const float a = unsafeAtomicAdd(sum + i, x);
+    const float e = two_sum_err(a, x, a + x);
+    if (e != 0.0f) unsafeAtomicAdd(comp + i, e);
+}
+
+// a compensated partial (s, c) into the pair (sum[i], comp[i])
+__device__ __forceinline__ void film_add_partial(float* __restrict__ sum, float* __restrict__ comp, size_t i, float s,
+                                                 float c)
+{
+    if (s != 0.0f) {
+        const float a = unsafeAtomicAdd(sum + i, s);
+        c += two_sum_err(a, s, a + s);
+    }
+    if (c != 0.0f) unsafeAtomicAdd(comp + i, c);
+}
+
+// x into the compensated pair (hs[i], hc[i]) of an LDS histogram
+__device__ __forceinline__ void lds_add(float* hs, float* hc, uint32_t i, float x)
+{
+    const float a = atomicAdd(hs + i, x);
+    const float e = two_sum_err(a, x, a + x);
+    if (e != 0.0f) atomicAdd(hc + i, e);
+}
+
+// film: bins the visible contributions of this bounce into the film. A block takes a range of
+// kFilmRange consecutive shadow records; their pixels are few (paths are enumerated pixel-major
 // and queue compaction keeps runs of neighbouring paths together), so each block accumulates into
-// LDS histograms of up to `pix_slots` pixels x T bins x RGB and flushes one float atomic per
-// non-zero bin; records of further pixels go straight to HBM atomics. Short ranges (512 records,
-// two 1024-bin pixels in 24 KB of LDS) measured best: 1.6x faster than 4096-record ranges in
-// 48 KB on the 1024-bin films, whose per-pixel records spread over many bins.
+// compensated LDS histograms of up to `pix_slots` pixels x T bins x RGB and flushes each non-zero
+// bin's (sum, comp) partial with one compensated add; records of further pixels go straight to
+// HBM. Short ranges (512 records, two 1024-bin pixels) measured best: 1.6x faster than 4096-record
+// ranges on the 1024-bin films, whose per-pixel records spread over many bins.
 // (StreakCameraFilm::add_samples / add_temporal_sample, StreakCameraFilm.h:139-192, 280-297)
 constexpr int kFilmBlock = 256;
 constexpr uint32_t kFilmRange = 512;
-constexpr uint32_t kFilmLdsBytes = 24 * 1024;
+constexpr uint32_t kFilmLdsBytes = 24 * 1024;  // per accumulator half (sums; as much again for comp)
 constexpr uint32_t kFilmMaxSlots = 8;
 
 template <bool GUIDED>
 __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev Wv, uint32_t pix_slots, float* contrib)
 {
-    extern __shared__ float hist[];          // pix_slots * T * 3
+    extern __shared__ float hist[];          // sums: pix_slots * T * 3, then as many compensations
     __shared__ uint32_t slot_pix[kFilmMaxSlots];
     const uint32_t count = Wv.counters[C_SHADOW];
     const uint32_t T = uint32_t(R.T), T3 = 3u * T, hsize = pix_slots * T3;
+    float* hcomp = hist + hsize;
     // the histograms start at zero; each flush reads and clears only the slots that were used
-    for (uint32_t i = threadIdx.x; i < hsize; i += blockDim.x) hist[i] = 0.0f;
+    for (uint32_t i = threadIdx.x; i < 2u * hsize; i += blockDim.x) hist[i] = 0.0f;
     for (uint32_t start = blockIdx.x * kFilmRange; start < count; start += gridDim.x * kFilmRange) {
         const uint32_t end = min(start + kFilmRange, count);
         if (threadIdx.x < kFilmMaxSlots) slot_pix[threadIdx.x] = 0xFFFFFFFFu;
@@ -1022,26 +1112,27 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
                 }
             }
             if (slot >= 0) {
-                float* h = hist + uint32_t(slot) * T3 + bin * 3u;
-                atomicAdd(h + 0, c.x);
-                atomicAdd(h + 1, c.y);
-                atomicAdd(h + 2, c.z);
+                const uint32_t h = uint32_t(slot) * T3 + bin * 3u;
+                lds_add(hist, hcomp, h + 0, c.x);
+                lds_add(hist, hcomp, h + 1, c.y);
+                lds_add(hist, hcomp, h + 2, c.z);
             } else {
-                float* dst = R.film + (size_t(pix) * T + bin) * 3;
-                unsafeAtomicAdd(dst + 0, c.x);
-                unsafeAtomicAdd(dst + 1, c.y);
-                unsafeAtomicAdd(dst + 2, c.z);
+                const size_t e = (size_t(pix) * T + bin) * 3;
+                film_add(R.film, R.film_comp, e + 0, c.x);
+                film_add(R.film, R.film_comp, e + 1, c.y);
+                film_add(R.film, R.film_comp, e + 2, c.z);
             }
         }
         __syncthreads();
         uint32_t used = 0;  // slots are claimed in order
         while (used < pix_slots && slot_pix[used] != 0xFFFFFFFFu) ++used;
         for (uint32_t i = threadIdx.x; i < used * T3; i += blockDim.x) {
-            float v = hist[i];
-            if (v != 0.0f) {
-                uint32_t k = i / T3;
-                unsafeAtomicAdd(R.film + size_t(slot_pix[k]) * T3 + (i - k * T3), v);
+            const float v = hist[i], vc = hcomp[i];
+            if (v != 0.0f || vc != 0.0f) {
+                const uint32_t k = i / T3;
+                film_add_partial(R.film, R.film_comp, size_t(slot_pix[k]) * T3 + (i - k * T3), v, vc);
                 hist[i] = 0.0f;
+                hcomp[i] = 0.0f;
             }
         }
         __syncthreads();
@@ -1049,7 +1140,7 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
 }
 
 // multibounce films: visible connections mark their layer for the camera sample, add to the
-// steady image, and (inside the time window) to their layer's pass film
+// steady image, and (inside the time window) to their layer's film
 __global__ void __launch_bounds__(kBlock) film_mb_kernel(RenderDev R, WaveDev Wv)
 {
     const uint32_t count = Wv.counters[C_SHADOW];
@@ -1057,16 +1148,15 @@ __global__ void __launch_bounds__(kBlock) film_mb_kernel(RenderDev R, WaveDev Wv
         if (!Wv.sh_vis[r]) continue;
         const float4 c = Wv.sh_c[r];
         const uint32_t tag = Wv.sh_tag[r], pix = __float_as_uint(c.w), fidx = __float_as_uint(Wv.sh_d[r].w);
-        atomicOr(R.cs_bits + (tag >> 5), 1u << (tag & 31u));
+        atomicOr(R.cs_bits + (tag >> kMaxLayersTagBits), 1u << (tag & ((1u << kMaxLayersTagBits) - 1u)));
         if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
-            unsafeAtomicAdd(R.steady + size_t(pix) * 3 + 0, c.x);
-            unsafeAtomicAdd(R.steady + size_t(pix) * 3 + 1, c.y);
-            unsafeAtomicAdd(R.steady + size_t(pix) * 3 + 2, c.z);
+            film_add(R.steady, R.steady_comp, size_t(pix) * 3 + 0, c.x);
+            film_add(R.steady, R.steady_comp, size_t(pix) * 3 + 1, c.y);
+            film_add(R.steady, R.steady_comp, size_t(pix) * 3 + 2, c.z);
             if (fidx != 0xFFFFFFFFu) {
-                float* dst = R.film + size_t(fidx) * 3;
-                unsafeAtomicAdd(dst + 0, c.x);
-                unsafeAtomicAdd(dst + 1, c.y);
-                unsafeAtomicAdd(dst + 2, c.z);
+                film_add(R.film, R.film_comp, size_t(fidx) * 3 + 0, c.x);
+                film_add(R.film, R.film_comp, size_t(fidx) * 3 + 1, c.y);
+                film_add(R.film, R.film_comp, size_t(fidx) * 3 + 2, c.z);
             }
         }
     }
@@ -1090,7 +1180,7 @@ __global__ void __launch_bounds__(kBlock) mb_weight_kernel(RenderDev R, WaveDev 
     }
 }
 
-// batched renders: visible contributions go straight to their job's pass film with float atomics
+// batched renders: visible contributions go straight to their job's film, one compensated add each
 __global__ void __launch_bounds__(kBlock) film_batch_kernel(const JobDev* __restrict__ jobs, WaveDev Wv)
 {
     const uint32_t count = Wv.counters[C_SHADOW];
@@ -1100,37 +1190,14 @@ __global__ void __launch_bounds__(kBlock) film_batch_kernel(const JobDev* __rest
         float4 c = Wv.sh_c[r];
         uint32_t pix = __float_as_uint(c.w);
         uint32_t bin = __float_as_uint(Wv.sh_d[r].w) - pix * uint32_t(R.T);
-        float* dst = R.film + (size_t(pix) * uint32_t(R.T) + bin) * 3;
-        unsafeAtomicAdd(dst + 0, c.x);
-        unsafeAtomicAdd(dst + 1, c.y);
-        unsafeAtomicAdd(dst + 2, c.z);
+        const size_t e = (size_t(pix) * uint32_t(R.T) + bin) * 3;
+        film_add(R.film, R.film_comp, e + 0, c.x);
+        film_add(R.film, R.film_comp, e + 1, c.y);
+        film_add(R.film, R.film_comp, e + 2, c.z);
     }
 }
 
 // ---------------------------------------------------------------------------------------------
-// film merge: Neumaier-compensated accumulation of the pass buffer into (sum, comp); clears the pass
-__global__ void __launch_bounds__(kBlock) film_merge_kernel(float4* __restrict__ pass, float4* __restrict__ sum,
-                                                            float4* __restrict__ comp, size_t n4)
-{
-    const size_t stride = size_t(gridDim.x) * blockDim.x;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        float4 x = pass[i];
-        if (x.x == 0.0f && x.y == 0.0f && x.z == 0.0f && x.w == 0.0f) continue;
-        float4 s = sum[i], c = comp[i];
-        float xs[4] = {x.x, x.y, x.z, x.w}, ss[4] = {s.x, s.y, s.z, s.w}, cs[4] = {c.x, c.y, c.z, c.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float t = ss[k] + xs[k];
-            if (fabsf(ss[k]) >= fabsf(xs[k])) cs[k] += (ss[k] - t) + xs[k];
-            else cs[k] += (xs[k] - t) + ss[k];
-            ss[k] = t;
-        }
-        sum[i] = make_float4(ss[0], ss[1], ss[2], ss[3]);
-        comp[i] = make_float4(cs[0], cs[1], cs[2], cs[3]);
-        pass[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-}
-
 // film read-back: one block per pixel writes (sum + comp) / weight of its T x RGB bins (0/0 -> 0,
 // ImageRow::weight, Image.h:153-162) in double precision, and the pixel's steady value (the sum
 // over its bins) for the steady image (StreakCameraFilm.h:369-397)
@@ -1235,6 +1302,7 @@ __global__ void __launch_bounds__(kTraceBlock) occluded_kernel(SceneDev S, const
         case kTravBinary: TR_STACK_DISPATCH3B(kTravBinary, bound, batch, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break; \
         case kTravWide: TR_STACK_DISPATCH3B(kTravWide, bound, batch, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break;     \
         case kTravWide16: TR_STACK_DISPATCH3B(kTravWide16, bound, batch, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break; \
+        case kTravWidePP: TR_STACK_DISPATCH3B(kTravWidePP, bound, batch, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break; \
         default: TR_STACK_DISPATCH3B(kTravWideKeyed, bound, batch, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break;       \
         }                                                                                                   \
     } while (0)
@@ -1330,11 +1398,11 @@ struct LaunchCfg {
 // launch flags -> stack format
 static int stack_fmt_of(uint32_t flags)
 {
-    return (flags & TR_RENDER_STACK32) ? 1 : (flags & TR_RENDER_STACK_KEYED) ? 2 : 0;
+    return (flags & TR_RENDER_POSTPONE_LEAVES) ? 3 : (flags & TR_RENDER_STACK32) ? 1 : (flags & TR_RENDER_STACK_KEYED) ? 2 : 0;
 }
 static uint32_t stack_flags_of(int fmt)
 {
-    return fmt == 1 ? TR_RENDER_STACK32 : fmt == 2 ? TR_RENDER_STACK_KEYED : 0u;
+    return fmt == 1 ? TR_RENDER_STACK32 : fmt == 2 ? TR_RENDER_STACK_KEYED : fmt == 3 ? TR_RENDER_POSTPONE_LEAVES : 0u;
 }
 
 struct WaveBuffers {
@@ -1348,6 +1416,10 @@ struct WaveBuffers {
     uint32_t* sh_job = nullptr;  // allocated by the first batched render
     WaveCtl* ctl = nullptr;
     JobDev* d_jobs = nullptr;
+    // pinned host staging of the job table: the upload is asynchronous, so its source must stay
+    // valid until the copy has run (every render ends with a stream synchronisation; a pageable
+    // vector freed right after hipMemcpyAsync may still be read by the runtime's copy)
+    JobDev* h_jobs = nullptr;
     uint32_t jobs_cap = 0;
     uint32_t* cs_bits = nullptr;  // multibounce films: layer marks per camera sample of a wavefront
     uint32_t cs_cap = 0;
@@ -1360,6 +1432,8 @@ struct WaveBuffers {
         }
         dev_free(hit); dev_free(sh_o); dev_free(sh_d); dev_free(sh_c); dev_free(sh_vis); dev_free(counters);
         dev_free(sh_tag); dev_free(sh_job); dev_free(ctl); dev_free(d_jobs); dev_free(cs_bits);
+        if (h_jobs) (void)hipHostFree(h_jobs);
+        h_jobs = nullptr;
         cap = 0;
         jobs_cap = 0;
         cs_cap = 0;
@@ -1378,8 +1452,22 @@ struct WaveBuffers {
         int rc;
         if (!sh_job && cap && (rc = dev_alloc(&sh_job, cap))) return rc;
         if (n_jobs + 1 > jobs_cap) {
+            // an earlier upload from the pinned table may still be queued (a render that failed
+            // before its final synchronisation): drain the device before freeing it
+            if (h_jobs) {
+                (void)hipDeviceSynchronize();
+                (void)hipHostFree(h_jobs);
+                h_jobs = nullptr;
+            }
             dev_free(d_jobs);
+            jobs_cap = 0;
             if ((rc = dev_alloc(&d_jobs, n_jobs + 1))) return rc;
+            if (hipHostMalloc(reinterpret_cast<void**>(&h_jobs), size_t(n_jobs + 1) * sizeof(JobDev), hipHostMallocDefault) !=
+                hipSuccess) {
+                h_jobs = nullptr;
+                dev_free(d_jobs);
+                return fail(TR_ERR_NOMEM, "hipHostMalloc: job table");
+            }
             jobs_cap = n_jobs + 1;
         }
         return TR_OK;
@@ -1497,9 +1585,10 @@ struct tr_film_s {
     int rows = 0;
     int layers = 0;      // multibounce bounce films (0: one streak film)
     size_t n_elems = 0;  // max(layers, 1) * rows*W*T*3
-    float *d_pass = nullptr, *d_sum = nullptr, *d_comp = nullptr;
-    float *d_layer_w = nullptr, *d_steady = nullptr;  // multibounce films only
+    float *d_sum = nullptr, *d_comp = nullptr;        // compensated sums: sum + comp = film
+    float *d_layer_w = nullptr, *d_steady = nullptr;  // multibounce films only (steady: sums, then comps)
     float *d_read = nullptr, *d_read_steady = nullptr;  // read-back staging (allocated by the first read)
+    size_t read_cap = 0;                                // floats d_read holds
     double* d_read_w = nullptr;
     std::vector<double> weight;  // rows*W
     hipStream_t stream = nullptr;
@@ -1908,13 +1997,12 @@ int tr_film_create(tr_film* out, const tr_film_desc* d)
     f->rows = rows;
     f->layers = layers;
     f->n_elems = n;
-    size_t n_alloc = (n + 3) / 4 * 4;  // merge kernel works on float4
     auto release = [&]() {
-        dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp); dev_free(f->d_layer_w); dev_free(f->d_steady);
+        dev_free(f->d_sum); dev_free(f->d_comp); dev_free(f->d_layer_w); dev_free(f->d_steady);
     };
-    if ((rc = dev_alloc(&f->d_pass, n_alloc)) || (rc = dev_alloc(&f->d_sum, n_alloc)) || (rc = dev_alloc(&f->d_comp, n_alloc)) ||
+    if ((rc = dev_alloc(&f->d_sum, n)) || (rc = dev_alloc(&f->d_comp, n)) ||
         (layers && ((rc = dev_alloc(&f->d_layer_w, size_t(layers) * rows * size_t(d->width))) ||
-                    (rc = dev_alloc(&f->d_steady, size_t(rows) * size_t(d->width) * 3))))) {
+                    (rc = dev_alloc(&f->d_steady, size_t(rows) * size_t(d->width) * 6))))) {
         release();
         delete f;
         return rc;
@@ -1938,7 +2026,7 @@ int tr_film_destroy(tr_film f)
 {
     if (!f) return TR_OK;
     if (f->stream) (void)hipStreamSynchronize(f->stream);
-    dev_free(f->d_pass); dev_free(f->d_sum); dev_free(f->d_comp); dev_free(f->d_layer_w); dev_free(f->d_steady);
+    dev_free(f->d_sum); dev_free(f->d_comp); dev_free(f->d_layer_w); dev_free(f->d_steady);
     dev_free(f->d_read); dev_free(f->d_read_steady); dev_free(f->d_read_w);
     if (f->stream) (void)hipStreamDestroy(f->stream);
     delete f;
@@ -1948,14 +2036,13 @@ int tr_film_destroy(tr_film f)
 int tr_film_clear(tr_film f)
 {
     if (!f) return fail(TR_ERR_INVALID, "tr_film_clear: null film");
-    size_t bytes = (f->n_elems + 3) / 4 * 4 * sizeof(float);
-    HIP_TRY(hipMemsetAsync(f->d_pass, 0, bytes, f->stream));
+    const size_t bytes = f->n_elems * sizeof(float);
     HIP_TRY(hipMemsetAsync(f->d_sum, 0, bytes, f->stream));
     HIP_TRY(hipMemsetAsync(f->d_comp, 0, bytes, f->stream));
     if (f->layers) {
         const size_t px = size_t(f->rows) * size_t(f->desc.width);
         HIP_TRY(hipMemsetAsync(f->d_layer_w, 0, size_t(f->layers) * px * sizeof(float), f->stream));
-        HIP_TRY(hipMemsetAsync(f->d_steady, 0, px * 3 * sizeof(float), f->stream));
+        HIP_TRY(hipMemsetAsync(f->d_steady, 0, px * 6 * sizeof(float), f->stream));
     }
     HIP_TRY(hipStreamSynchronize(f->stream));
     std::fill(f->weight.begin(), f->weight.end(), 0.0);
@@ -1986,28 +2073,47 @@ int tr_film_read_sums(tr_film f, double* sums, double* weights)
     return TR_OK;
 }
 
+int tr_film_read_rows(tr_film f, int32_t row_begin, int32_t row_count, float* out, float* steady)
+{
+    if (!f) return fail(TR_ERR_INVALID, "tr_film_read_rows: null film");
+    if (f->layers) return fail(TR_ERR_INVALID, "tr_film_read_rows: multibounce films are read whole (tr_film_read)");
+    if (row_begin < 0 || row_count < 0 || row_begin + int64_t(row_count) > f->rows)
+        return fail(TR_ERR_INVALID, "tr_film_read_rows: rows outside the film");
+    if ((!out && !steady) || row_count == 0) return TR_OK;
+    const size_t W = size_t(f->desc.width), T = size_t(f->desc.time_bins), P = size_t(f->rows) * W;
+    const size_t p0 = size_t(row_begin) * W, np = size_t(row_count) * W, n = np * T * 3;
+    // normalised on the device; one copy of the result instead of both raw arrays
+    int rc;
+    if (!f->d_read_w && (rc = dev_alloc(&f->d_read_w, P))) return rc;
+    if (out && f->read_cap < n) {
+        dev_free(f->d_read);
+        f->read_cap = 0;
+        if ((rc = dev_alloc(&f->d_read, n))) return rc;
+        f->read_cap = n;
+    }
+    if (steady && !f->d_read_steady && (rc = dev_alloc(&f->d_read_steady, P * 3))) return rc;
+    HIP_TRY(hipMemcpyAsync(f->d_read_w, f->weight.data(), P * sizeof(double), hipMemcpyHostToDevice, f->stream));
+    film_normalize_kernel<<<uint32_t(np), kNormBlock, 0, f->stream>>>(f->d_sum + p0 * T * 3, f->d_comp + p0 * T * 3,
+                                                                     f->d_read_w + p0, uint32_t(3 * T),
+                                                                     out ? f->d_read : nullptr,
+                                                                     steady ? f->d_read_steady : nullptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && out) e = hipMemcpyAsync(out, f->d_read, n * sizeof(float), hipMemcpyDeviceToHost, f->stream);
+    if (e == hipSuccess && steady)
+        e = hipMemcpyAsync(steady, f->d_read_steady, np * 3 * sizeof(float), hipMemcpyDeviceToHost, f->stream);
+    // the weight upload reads f->weight asynchronously: always wait
+    const hipError_t se = hipStreamSynchronize(f->stream);
+    if (e == hipSuccess) e = se;
+    if (e != hipSuccess) return fail(TR_ERR_DEVICE, std::string("tr_film_read: ") + hipGetErrorString(e));
+    return TR_OK;
+}
+
 int tr_film_read(tr_film f, float* out, float* steady)
 {
     if (!f) return fail(TR_ERR_INVALID, "tr_film_read: null film");
     if (!out && !steady) return TR_OK;
     const size_t W = size_t(f->desc.width), T = size_t(f->desc.time_bins), P = size_t(f->rows) * W;
-    if (!f->layers) {
-        // normalised on the device; one copy of the result instead of both raw arrays
-        int rc;
-        if (!f->d_read_w && (rc = dev_alloc(&f->d_read_w, P))) return rc;
-        if (out && !f->d_read && (rc = dev_alloc(&f->d_read, f->n_elems))) return rc;
-        if (steady && !f->d_read_steady && (rc = dev_alloc(&f->d_read_steady, P * 3))) return rc;
-        HIP_TRY(hipMemcpyAsync(f->d_read_w, f->weight.data(), P * sizeof(double), hipMemcpyHostToDevice, f->stream));
-        film_normalize_kernel<<<uint32_t(P), kNormBlock, 0, f->stream>>>(f->d_sum, f->d_comp, f->d_read_w,
-                                                                        uint32_t(3 * T), out ? f->d_read : nullptr,
-                                                                        steady ? f->d_read_steady : nullptr);
-        HIP_TRY(hipGetLastError());
-        if (out) HIP_TRY(hipMemcpyAsync(out, f->d_read, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
-        if (steady)
-            HIP_TRY(hipMemcpyAsync(steady, f->d_read_steady, P * 3 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
-        HIP_TRY(hipStreamSynchronize(f->stream));
-        return TR_OK;
-    }
+    if (!f->layers) return tr_film_read_rows(f, 0, f->rows, out, steady);
     std::vector<float> s(f->n_elems), c(f->n_elems);
     HIP_TRY(hipMemcpyAsync(s.data(), f->d_sum, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
     HIP_TRY(hipMemcpyAsync(c.data(), f->d_comp, f->n_elems * sizeof(float), hipMemcpyDeviceToHost, f->stream));
@@ -2015,7 +2121,7 @@ int tr_film_read(tr_film f, float* out, float* steady)
     {
         // multibounce films: each bounce film divided by its own weights; the steady image holds
         // every sample
-        std::vector<float> lw(size_t(f->layers) * P), st(P * 3);
+        std::vector<float> lw(size_t(f->layers) * P), st(P * 6);
         HIP_TRY(hipMemcpyAsync(lw.data(), f->d_layer_w, lw.size() * sizeof(float), hipMemcpyDeviceToHost, f->stream));
         HIP_TRY(hipMemcpyAsync(st.data(), f->d_steady, st.size() * sizeof(float), hipMemcpyDeviceToHost, f->stream));
         HIP_TRY(hipStreamSynchronize(f->stream));
@@ -2032,7 +2138,7 @@ int tr_film_read(tr_film f, float* out, float* steady)
         if (steady)
             for (size_t p = 0; p < P; ++p)
                 for (int k = 0; k < 3; ++k) {
-                    double v = double(st[p * 3 + size_t(k)]) / f->weight[p];
+                    double v = (double(st[p * 3 + size_t(k)]) + double(st[(P + p) * 3 + size_t(k)])) / f->weight[p];
                     steady[p * 3 + size_t(k)] = std::isfinite(v) ? float(v) : 0.0f;
                 }
         return TR_OK;
@@ -2091,6 +2197,85 @@ int tr_film_device_buffers(tr_film f, float** d_sum, float** d_comp)
     return TR_OK;
 }
 
+// StreakCameraFilm::add_samples for caller-made samples: binned on the host as the reference bins
+// (pos_sensor = (time - offset) / exposure, kept if 0 <= pos_sensor < T; StreakCameraFilm.h:165-171),
+// then deposited by the renders' own film kernel from shadow-record-shaped staging buffers.
+int tr_film_add_samples(tr_film f, const tr_film_sample* samples, uint32_t n)
+{
+    if (!f) return fail(TR_ERR_INVALID, "tr_film_add_samples: null film");
+    if (f->layers) return fail(TR_ERR_INVALID, "tr_film_add_samples: multibounce films take samples from renders only");
+    if (n == 0) return TR_OK;
+    if (!samples) return fail(TR_ERR_INVALID, "tr_film_add_samples: null samples");
+    if (n > (1u << 31)) return fail(TR_ERR_INVALID, "tr_film_add_samples: more than 2^31 samples in one call");
+    const tr_film_desc& fd = f->desc;
+    const uint32_t W = uint32_t(fd.width), T = uint32_t(fd.time_bins);
+    const int row0 = fd.scanline >= 0 ? fd.scanline : 0;
+    std::vector<float4> hc(n), hd(n);
+    std::vector<uint8_t> hv(n);
+    std::vector<double> wadd(f->weight.size(), 0.0);
+    auto bits = [](uint32_t u) {
+        float v;
+        std::memcpy(&v, &u, sizeof v);
+        return v;
+    };
+    for (uint32_t i = 0; i < n; ++i) {
+        const tr_film_sample& q = samples[i];
+        const int row = q.y - row0;
+        if (q.x < 0 || uint32_t(q.x) >= W || row < 0 || row >= f->rows)
+            return fail(TR_ERR_INVALID, "tr_film_add_samples: sample " + std::to_string(i) + " lies outside the film");
+        const uint32_t pix = uint32_t(row) * W + uint32_t(q.x);
+        wadd[pix] += double(q.weight);
+        const float sensor = (q.time - fd.offset) / fd.exposure;
+        const bool in_window = sensor < float(T) && sensor >= 0.0f;
+        const uint32_t bin = in_window ? std::min(uint32_t(std::floor(sensor)), T - 1u) : 0u;
+        hc[i] = make_float4(q.rgb[0], q.rgb[1], q.rgb[2], bits(pix));
+        hd[i] = make_float4(0.0f, 0.0f, 0.0f, bits(pix * T + bin));
+        hv[i] = in_window ? 1 : 0;
+    }
+    float4 *dc = nullptr, *dd = nullptr;
+    uint8_t* dv = nullptr;
+    uint32_t* dcount = nullptr;
+    auto release = [&]() {
+        dev_free(dc); dev_free(dd); dev_free(dv); dev_free(dcount);
+    };
+    int rc;
+    if ((rc = dev_alloc(&dc, n)) || (rc = dev_alloc(&dd, n)) || (rc = dev_alloc(&dv, n)) || (rc = dev_alloc(&dcount, C_WORDS))) {
+        release();
+        return rc;
+    }
+    uint32_t counts[C_WORDS] = {};
+    counts[C_SHADOW] = n;
+    RenderDev R{};
+    R.film = f->d_sum;
+    R.film_comp = f->d_comp;
+    R.W = fd.width;
+    R.H = fd.height;
+    R.T = fd.time_bins;
+    WaveDev Wv{};
+    Wv.sh_c = dc;
+    Wv.sh_d = dd;
+    Wv.sh_vis = dv;
+    Wv.counters = dcount;
+    const uint32_t pix_slots = std::min<uint32_t>(kFilmMaxSlots, kFilmLdsBytes / (T * 12u));
+    const size_t lds = 2u * size_t(pix_slots) * size_t(T) * 12u;
+    const uint32_t fg = std::max(1u, std::min((n + kFilmRange - 1) / kFilmRange, uint32_t(tri::compute_units()) * 4u));
+    hipError_t e = hipMemcpyAsync(dc, hc.data(), size_t(n) * sizeof(float4), hipMemcpyHostToDevice, f->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dd, hd.data(), size_t(n) * sizeof(float4), hipMemcpyHostToDevice, f->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dv, hv.data(), size_t(n), hipMemcpyHostToDevice, f->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dcount, counts, sizeof counts, hipMemcpyHostToDevice, f->stream);
+    if (e == hipSuccess) {
+        film_kernel<false><<<fg, kFilmBlock, lds, f->stream>>>(R, Wv, pix_slots, nullptr);
+        e = hipGetLastError();
+    }
+    // the staging vectors above are read by the copies: wait before they go out of scope
+    const hipError_t se = hipStreamSynchronize(f->stream);
+    if (e == hipSuccess) e = se;
+    release();
+    if (e != hipSuccess) return fail(TR_ERR_DEVICE, std::string("tr_film_add_samples: ") + hipGetErrorString(e));
+    for (size_t p = 0; p < wadd.size(); ++p) f->weight[p] += wadd[p];
+    return TR_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -2144,7 +2329,8 @@ int make_job(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* 
     nrm(vv);
     float fov = float(M_PI / 180. * double(cam->fov_deg) / 2);
     RenderDev R{};
-    R.film = f->d_pass;
+    R.film = f->d_sum;
+    R.film_comp = f->d_comp;
     R.W = fd.width;
     R.H = fd.height;
     R.T = fd.time_bins;
@@ -2175,6 +2361,7 @@ int make_job(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_desc* 
     R.layers = f->layers;
     R.layer_stride = uint32_t(size_t(f->rows) * size_t(fd.width) * size_t(fd.time_bins));
     R.steady = f->d_steady;
+    R.steady_comp = f->d_steady ? f->d_steady + size_t(f->rows) * size_t(fd.width) * 3 : nullptr;
     R.layer_w = f->d_layer_w;
     R.cs_bits = nullptr;  // set by the launch (wavefront buffers)
     out->scene = s;
@@ -2322,6 +2509,8 @@ struct Launch {
     }
 };
 
+static_assert(kMaxLayersTagBits == kMbLayerTagBits, "layer tag width");
+
 // Renders every job of `jobs` in one sequence of wavefronts on `stream`; jobs share wavefronts
 // when there are several (batched kernels look up each path's job).
 int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, const LaunchCfg& cfg, tr_guide guide,
@@ -2329,6 +2518,12 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
              std::chrono::steady_clock::time_point t_start)
 {
     if (jobs.empty()) return fail(TR_ERR_INVALID, "render: no job");
+    // every return drains the stream, so no queued copy or kernel outlives the host buffers and
+    // device views it was given (the normal path synchronises as well, to read the counters)
+    struct Drain {
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{stream};
     const bool batch = jobs.size() > 1;
     if (batch && guide) return fail(TR_ERR_INVALID, "render: guided renders take one job");
     uint64_t total = 0;
@@ -2343,7 +2538,8 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
     }
     // 16-bit stack entries hold node indices below 65,536
     const bool small = max_nodes4 <= 65536u && cfg.stack_fmt != 1;
-    const int trav_ext = !cfg.wide ? kTravBinary : !small ? kTravWide : cfg.stack_fmt == 2 ? kTravWideKeyed : kTravWide16;
+    const int trav_ext = !cfg.wide ? kTravBinary : !small ? kTravWide : cfg.stack_fmt == 2 ? kTravWideKeyed
+                                                                      : cfg.stack_fmt == 3 ? kTravWidePP : kTravWide16;
     const int trav_sh = !cfg.wide ? kTravBinary : !small ? kTravWide : kTravWide16;
     const bool mb = jobs[0].dev.R.layers > 0;
     for (auto& j : jobs)
@@ -2352,9 +2548,11 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
     LaunchCfg c = cfg;
     c.wave = uint32_t(std::min<uint64_t>(c.wave ? c.wave : kDefaultWave, std::max<uint64_t>(total, 1)));
     if (mb) {
-        // wavefronts hold whole camera samples, whose layer marks are counted per wavefront
+        // wavefronts hold whole camera samples, whose layer marks are counted per wavefront; a
+        // shadow record tags its camera sample within the wavefront in 27 bits (and its layer in 5),
+        // so a wavefront holds at most kMbMaxCameraSamples camera samples
         const uint32_t ni = uint32_t(jobs[0].dev.R.n_inc);
-        c.wave = std::max(ni, c.wave / ni * ni);
+        c.wave = mb_wave(c.wave, ni);
     }
     int rc = wb.ensure(c.wave);
     if (rc) return rc;
@@ -2387,11 +2585,13 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
     }
     if (batch) {
         if ((rc = wb.ensure_batch(uint32_t(jobs.size())))) return rc;
-        std::vector<JobDev> dj(jobs.size() + 1);
+        // the previous render on this stream ended with a synchronisation, so the pinned table is
+        // free to overwrite
+        JobDev* dj = wb.h_jobs;
         for (size_t k = 0; k < jobs.size(); ++k) dj[k] = jobs[k].dev;
         dj[jobs.size()] = jobs.back().dev;
         dj[jobs.size()].path_begin = total;
-        HIP_TRY(hipMemcpyAsync(wb.d_jobs, dj.data(), dj.size() * sizeof(JobDev), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(wb.d_jobs, dj, (jobs.size() + 1) * sizeof(JobDev), hipMemcpyHostToDevice, stream));
     }
     if (c.prof && !c.graph) {
         if (!*ev_made) {
@@ -2401,11 +2601,12 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
         L.events_on = true;
         L.extend_only = c.prof_extend;
     }
-    // LDS histogram slots of the film kernel: as many pixels x T bins x RGB as fit in kFilmLdsBytes
+    // LDS histogram slots of the film kernel: as many pixels x T bins x RGB as fit in kFilmLdsBytes,
+    // with a compensation beside each sum
     {
         const uint32_t T = uint32_t(jobs[0].dev.R.T);
         L.pix_slots = std::min<uint32_t>(kFilmMaxSlots, kFilmLdsBytes / (T * 12u));
-        L.film_lds = size_t(L.pix_slots) * size_t(T) * 12u;
+        L.film_lds = 2u * size_t(L.pix_slots) * size_t(T) * 12u;
     }
     HIP_TRY(hipMemsetAsync(wb.counters, 0, C_WORDS * sizeof(uint32_t), stream));
     // pass buffers are written on this stream; order it after any pending film work
@@ -2467,19 +2668,6 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
             }
         }
     }
-    // each distinct film: pass buffer -> compensated sum
-    std::vector<tr_film> merged;
-    for (auto& j : jobs) {
-        if (std::find(merged.begin(), merged.end(), j.film) != merged.end()) continue;
-        merged.push_back(j.film);
-        tr_film f = j.film;
-        size_t n4 = (f->n_elems + 3) / 4;
-        uint32_t mg = uint32_t(std::max<size_t>(1, std::min<size_t>((n4 + kBlock - 1) / kBlock, 256u * 64u)));
-        if ((rc = L.begin(TR_K_FILM_MERGE))) break;
-        film_merge_kernel<<<mg, kBlock, 0, stream>>>(reinterpret_cast<float4*>(f->d_pass), reinterpret_cast<float4*>(f->d_sum),
-                                                       reinterpret_cast<float4*>(f->d_comp), n4);
-        if ((rc = L.end())) break;
-    }
     uint32_t counters[C_WORDS];
     hipError_t e = hipMemcpyAsync(counters, wb.counters, sizeof counters, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -2520,7 +2708,7 @@ int render_impl(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_des
     c.graph = (d->flags & TR_RENDER_GRAPH) != 0;
     c.prof = (d->flags & (TR_RENDER_PROFILE | TR_RENDER_PROFILE_EXTEND)) != 0;
     c.prof_extend = (d->flags & TR_RENDER_PROFILE) == 0 && (d->flags & TR_RENDER_PROFILE_EXTEND) != 0;
-    if (d->flags & (TR_RENDER_STACK32 | TR_RENDER_STACK_KEYED)) c.stack_fmt = stack_fmt_of(d->flags);
+    if (d->flags & (TR_RENDER_STACK32 | TR_RENDER_STACK_KEYED | TR_RENDER_POSTPONE_LEAVES)) c.stack_fmt = stack_fmt_of(d->flags);
     c.mode = guide ? kShadeGuided : ((d->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef);
     return run_jobs(s->stream, s->wave, jobs, c, guide, s->compute_units, s->ev, &s->ev_made, stats, t_start);
 }
@@ -2759,7 +2947,7 @@ int tr_autotune(tr_batch b, tr_launch_desc* best, double* ms_best)
         if (ms[1] < ms[0] * kTuneMargin) bestc.per_job = true;
     }
     // stage 4: traversal stack formats
-    for (int fmt : {1, 2}) {
+    for (int fmt : {1, 2, 3}) {
         LaunchCfg c = bestc;
         if (!c.wide || c.stack_fmt == fmt) continue;
         c.stack_fmt = fmt;

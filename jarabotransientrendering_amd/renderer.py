@@ -288,6 +288,15 @@ class StreakCameraFilm:
         nat.check(nat.lib().tr_film_read(self.handle, nat.as_ptr(out), nat.as_ptr(st) if steady else None))
         return (out, st) if steady else out
 
+    def read_rows(self, row_begin: int, row_count: int, steady: bool = False):
+        """Normalised streak data of rows [row_begin, row_begin + row_count) only: [rows][W][T][3]
+        (and their steady image [rows][W][3])."""
+        out = np.empty((row_count, self.width, self.time_bins, 3), dtype=np.float32)
+        st = np.empty((row_count, self.width, 3), dtype=np.float32) if steady else None
+        nat.check(nat.lib().tr_film_read_rows(self.handle, int(row_begin), int(row_count), nat.as_ptr(out),
+                                              nat.as_ptr(st) if steady else None))
+        return (out, st) if steady else out
+
     def read_sums(self):
         """Compensated sums [rows][W][T][3] (float64) and weights [rows][W] (multibounce films:
         a leading layer axis on both)."""
@@ -295,6 +304,13 @@ class StreakCameraFilm:
         w = np.empty(self._lead() + (self.rows, self.width), dtype=np.float64)
         nat.check(nat.lib().tr_film_read_sums(self.handle, nat.as_ptr(s), nat.as_ptr(w)))
         return s, w
+
+    def add_samples(self, samples: np.ndarray) -> None:
+        """StreakCameraFilm::add_samples (StreakCameraFilm.h:139-192) for caller-made samples: a
+        structured array of dtype FILM_SAMPLE_DTYPE (x, y, time, rgb, weight), deposited with
+        compensated additions by the renders' film kernel."""
+        a = np.ascontiguousarray(samples, dtype=nat.FILM_SAMPLE_DTYPE)
+        nat.check(nat.lib().tr_film_add_samples(self.handle, nat.as_ptr(a), a.shape[0]))
 
     def write(self, name: str, extension: str = "hdr") -> List[str]:
         """StreakCameraFilm::write + Film::write (StreakCameraFilm.h:233-242, 357-437; Film.h:358-471):

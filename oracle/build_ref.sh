@@ -14,3 +14,7 @@ mkdir -p "$HERE/_ref"
 g++ -std=c++14 -O2 -w -I "$REF/src" -isystem "$REF/src/External" \
     "$HERE/ref_harness.cpp" -o "$HERE/_ref/ref_harness" -lpthread
 echo "built $HERE/_ref/ref_harness"
+# the reference's command-line parser (bunnykiller.cpp, with its vendored TinyXML for -parse-xml-file)
+g++ -std=c++14 -O1 -w -I "$REF/src" -isystem "$REF/src/External" \
+    "$HERE/ref_cli_probe.cpp" "$REF/src/External/TinyXML/tinyxml2.cpp" -o "$HERE/_ref/ref_cli_probe" -lpthread
+echo "built $HERE/_ref/ref_cli_probe"

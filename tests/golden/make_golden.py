@@ -18,6 +18,9 @@ scenes below and records:
                                  un-normalised slices [B][W][T][3] and weights [B][W]
   image_<name>.npz               an RGB float image and the Radiance .hdr bytes the
                                  reference's Imaging::save wrote for it
+  cli_parse.json                 command lines and what the reference's parse_args and
+                                 main's pre-render wiring made of them (oracle/_ref/
+                                 ref_cli_probe); the meshes they name are cli_*.obj here
   manifest.json                  scene parameters, harness mode, seeds, and a
                                  SHA-256 of each scene's FP32 triangles
 
@@ -43,6 +46,42 @@ sys.path.insert(0, ROOT)
 from jarabotransientrendering_amd import scenes  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+CLI_PROBE = os.path.join(ROOT, "oracle", "_ref", "ref_cli_probe")
+
+# Command lines pinned against the reference's parser (run from this directory, so the mesh paths
+# are relative). "defaults" names nothing but a mesh: every other setting is ProgramParams' default.
+# (-name-mesh is never the last argument: the reference then reads argv[argc] while probing for a
+# material flag, bunnykiller.cpp:328, and crashes.)
+CLI_CASES = {
+    "defaults": ["-name-mesh", "cli_tri.obj", "-transient-state"],
+    "box_point": ["-film-name", "out/box", "-film-size-x", "64", "-film-size-t", "256", "-film-exposure", "0.01",
+                  "-camera-position", "0.5", "0.5", "-1.2", "-camera-focus", "0.5", "0.5", "0.5",
+                  "-camera-up", "0", "2", "0", "-camera-fov", "45", "-camera-spp", "4",
+                  "-bidirectional-path-tracing", "1", "-max-nb-bounces", "4", "-new-seed", "7",
+                  "-point-light-source", "0.5", "0.95", "0.5", "10", "10", "10",
+                  "-name-mesh", "cli_box.obj", "-lambertian", "0.6", "0.1", "0.2",
+                  "-name-mesh", "cli_tri.obj", "-two-sided", "-lambertian", "0.75", "0.75", "0.75",
+                  "-name-mesh", "cli_tri.obj", "-lambertian-rho", "0.45"],
+    "media_rect": ["-film-size-x", "512", "-film-size-y", "512", "-film-size-t", "1024", "-film-exposure", "0.01",
+                   "-film-offset", "0.25", "-film-scanline", "256", "-camera-unwarp", "-multibounce-streak", "4",
+                   "-camera-position", "0.5", "0.5", "-1", "-camera-focus", "0.5", "0.4", "0.5",
+                   "-camera-view-plane", "1.5", "-homogeneous-medium", "0.05", "0.05", "0.05", "0.5", "0.5", "0.5",
+                   "-pf-isotropic", "-time-sampling", "-scattering-level", "all", "-log-name", "media.log",
+                   "-name-mesh", "cli_box.obj", "-max-nb-bounces", "16",
+                   "-rectangular-area-light-source", "0.65", "0.99", "0.65", "0.65", "0.5", "0.65", "0.3", "0.25",
+                   "5", "-skipped-by-the-light", "-bidirectional-path-tracing", "3"],
+}
+
+
+def write_cli():
+    """cli_parse.json: each command line of CLI_CASES and the reference probe's JSON for it."""
+    out = {}
+    for tag, args in CLI_CASES.items():
+        res = subprocess.check_output([CLI_PROBE, *args], text=True, cwd=HERE)
+        out[tag] = {"args": args, "parsed": json.loads(res)}
+        print("cli", tag, len(res), "bytes")
+    with open(os.path.join(HERE, "cli_parse.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
 
 
 def scene_hash(s) -> str:
@@ -181,6 +220,9 @@ def main(only=None):
     if only == "images":
         write_images()
         return
+    if only == "cli":
+        write_cli()
+        return
     if only is not None and only.startswith("film:"):
         tag = only.split(":", 1)[1]
         with open(os.path.join(HERE, "manifest.json")) as fh:
@@ -243,6 +285,7 @@ def main(only=None):
             print(tag, "isect", int((hits[:, 0] >= 0).sum()), "hits")
     write_images()
     write_mb(manifest)
+    write_cli()
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, default=float)
 

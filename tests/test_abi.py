@@ -64,6 +64,7 @@ def test_struct_layouts_match_the_header():
     assert ctypes.sizeof(nat.FilmDesc) == 32
     assert ctypes.sizeof(nat.CameraDesc) == 44
     assert ctypes.sizeof(nat.RenderDesc) == 32
+    assert ctypes.sizeof(nat.FilmSample) == 32 == nat.FILM_SAMPLE_DTYPE.itemsize
     assert ctypes.sizeof(nat.RenderStats) == 8 * 4 + 4 * 2 + 8 + 8 * 8 + 4 * 8
 
 

@@ -155,3 +155,50 @@ def test_bunnykiller_command_line_matches_python_host(tmp_path, kind):
         # same streams, RGBE quantisation (8 mantissa bits), float-atomic order
         assert np.all(np.abs(f - img[i]) <= m / 100 + 1e-6 * img.max()), i
     assert (tmp_path / "out.hdr").exists() and (tmp_path / "out.log").exists()
+
+
+# ---------------------------------------------------------------------------------------------
+# the command line against the reference's own parser (tests/golden/cli_parse.json, written by
+# oracle/ref_cli_probe.cpp from the reference's parse_args and main)
+
+def _close(a, b, tol=1e-6):
+    return np.allclose(np.asarray(a, np.float64), np.asarray(b, np.float64), rtol=tol, atol=tol)
+
+
+@pytest.mark.parametrize("tag", ["defaults", "box_point", "media_rect"])
+def test_bunnykiller_parses_like_the_reference(tag):
+    """Film, camera, sampler, integrator, lights (with main's default light), medium and meshes
+    (material, two-sided flag, triangle count) as the reference's parse_args and main set them up,
+    for a command line that relies on ProgramParams' defaults and two that set most flags. The
+    command lines include the reference's quirks: -film-size-x sets the height too, the last
+    -lambertian-rho is the default material's albedo for every mesh, the rectangle light steps over
+    the argument after it, seeds are read with atoi."""
+    from tests.conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "cli_parse.json")) as fh:
+        case = json.load(fh)[tag]
+    r = _bk([*case["args"], "--print-parsed"], GOLDEN)
+    assert r.returncode == 0, r.stderr
+    mine, ref = json.loads(r.stdout), case["parsed"]
+    for k, v in ref["film"].items():
+        assert (_close(mine["film"][k], v) if isinstance(v, float) else mine["film"][k] == v), (k, mine["film"][k], v)
+    for k in ("position", "focus", "up", "view_plane_dist"):
+        assert _close(mine["camera"][k], ref["camera"][k]), k
+    assert mine["sampler"] == ref["sampler"]
+    assert mine["integrator"] == ref["integrator"]
+    assert len(mine["lights"]) == len(ref["lights"])
+    for a, b in zip(mine["lights"], ref["lights"]):
+        assert a["type"] == b["type"]
+        assert _close(a["position"], b["position"]) and _close(a["intensity"], b["intensity"])
+        if a["type"] == "rect":
+            d = np.asarray(a["target"], np.float32) - np.asarray(a["position"], np.float32)
+            assert _close(d / np.linalg.norm(d), b["normal"])
+            assert _close(a["size"], b["size"])
+    assert (mine["medium"] is None) == (ref["medium"] is None)
+    if ref["medium"] is not None:
+        assert _close(mine["medium"]["sigma_a"], ref["medium"]["sigma_a"])
+        assert _close(mine["medium"]["sigma_s"], ref["medium"]["sigma_s"])
+    assert len(mine["meshes"]) == len(ref["meshes"])
+    for a, b in zip(mine["meshes"], ref["meshes"]):
+        for k in ("file", "triangles", "two_sided", "default_material"):
+            assert a[k] == b[k], (k, a, b)
+        assert _close(a["kd"], b["kd"]), (a, b)

@@ -382,3 +382,22 @@ def test_binary_and_wide_bvh_render_the_same(R):
     b, sb, _ = R.render_scene(s, binary_bvh=True)
     assert sa["extension_rays"] == sb["extension_rays"] and sa["contributions"] == sb["contributions"]
     np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("flag", ["TR_RENDER_STACK32", "TR_RENDER_STACK_KEYED", "TR_RENDER_POSTPONE_LEAVES"])
+def test_traversal_launch_formats_render_the_same(R, flag):
+    """The closest-hit traversal variants (32-bit stack, keyed stack, postponed leaves) find the
+    same hits as the default (16-bit stack, leaves tested in the node step): same extension and
+    shadow ray counts, and the same film up to float summation order."""
+    from jarabotransientrendering_amd import _native as nat
+    s = scenes.icosphere_box(level=5, width=16, height=16, time_bins=256, sqrt_spp=3, n_incoming=2, max_bounces=6,
+                             seed=23)
+    base, sb, _ = R.render_scene(s)
+    w, cam, sampler, film, integ = R.setup_scene(s)
+    eng = R.TransientRenderer()
+    eng.extra_flags = getattr(nat, flag)
+    st = eng.render(None, w, integ, cam, film, sampler, seed=s.render.seed)
+    img = film.read()
+    assert st["extension_rays"] == sb["extension_rays"] and st["shadow_rays"] == sb["shadow_rays"]
+    assert st["contributions"] == sb["contributions"]
+    np.testing.assert_allclose(img, base, rtol=1e-5, atol=1e-7)

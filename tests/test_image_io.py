@@ -1,6 +1,6 @@
-"""Image output (tr_image_save / tr_film_write): Radiance RGBE files decode to the same pixels as
-the files the reference's Imaging::save wrote for the same float images (tests/golden/image_*.npz),
-raw dumps are the float data, and the decoder used here is pinned by the reference's own bytes."""
+"""Image output (tr_image_save / tr_film_write): Radiance RGBE files are byte for byte the files
+the reference's Imaging::save wrote for the same float images (tests/golden/image_*.npz), raw dumps
+are the float data, and the decoder used here is pinned by the reference's own bytes."""
 import os
 
 import numpy as np
@@ -56,7 +56,9 @@ def test_rgbe_matches_reference_imaging_save(tmp_path, tag):
     ref_q, ref_f = decode_rgbe(g["hdr"].tobytes())
     path = str(tmp_path / "out.hdr")
     assert nat.lib().tr_image_save(path.encode(), nat.as_ptr(img), W, H) == 0
-    mine_q, mine_f = decode_rgbe(open(path, "rb").read())
+    mine = open(path, "rb").read()
+    assert mine == g["hdr"].tobytes()          # the reference's file, byte for byte
+    mine_q, mine_f = decode_rgbe(mine)
     assert mine_q.shape == (H, W, 4)
     np.testing.assert_array_equal(mine_q, ref_q)
     # and the decoded values are the (non-negative) inputs to RGBE precision: 8 mantissa bits of

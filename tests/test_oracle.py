@@ -394,3 +394,25 @@ def test_oracle_multibounce_matches_reference(oracle, golden_manifest, tag):
     f1, w1, _ = oracle.render(s, oracle.RNG_REFERENCE, eps=oracle.NON_EMBREE_EPS)
     np.testing.assert_allclose(f.sum(axis=0), f1, rtol=1e-12, atol=1e-12)
     assert np.array_equal(w, w1)
+
+
+@pytest.mark.parametrize("rng_mode", ["reference", "per_path"])
+def test_non_finite_samples_are_zeroed(oracle, rng_mode):
+    """TransientRenderer.h:167-171: a sample whose radiance has a non-finite channel becomes 0 and
+    the camera sample's weight still counts. A second light of infinite (or NaN) intensity then adds
+    nothing: the film equals the film with that light at zero intensity on the same draws, and its
+    weights are the usual ones."""
+    mode = oracle.RNG_REFERENCE if rng_mode == "reference" else oracle.RNG_PER_PATH
+    films = {}
+    for label, val in (("inf", np.inf), ("nan", np.nan), ("zero", 0.0)):
+        s = scenes.cornell_box(width=6, height=6, time_bins=64, exposure=0.04, sqrt_spp=2, max_bounces=3, seed=4)
+        # a non-finite red channel makes the whole sample non-finite; "zero" is a black light
+        extra = np.array([[0.3, 0.9, 0.4, val, val != 0, val != 0]], dtype=np.float32)
+        s.point_lights = np.concatenate([s.point_lights, extra]).astype(np.float32)
+        f, w, _ = oracle.render(s, mode)
+        films[label] = (f, w)
+    assert np.isfinite(films["inf"][0]).all() and np.isfinite(films["nan"][0]).all()
+    np.testing.assert_array_equal(films["inf"][0], films["zero"][0])
+    np.testing.assert_array_equal(films["nan"][0], films["zero"][0])
+    np.testing.assert_array_equal(films["inf"][1], np.full((6, 6), 4.0))
+    assert films["zero"][0].sum() > 0
