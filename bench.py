@@ -560,8 +560,10 @@ def main():
                     help="only run device-to-device copies of BYTES (counter calibration)")
     ap.add_argument("--guide-mlp", action="store_true",
                     help="only measure the guiding network's kernels (evaluation, training, guided shading)")
-    ap.add_argument("--guided", choices=["on", "off", "only"], default="off",
-                    help="add the guided-versus-plain time-to-relMSE comparison (only: print just that)")
+    ap.add_argument("--guided", choices=["quick", "on", "off", "only"], default="quick",
+                    help="guided-versus-plain time-to-relMSE comparison in the line: quick (default; one "
+                         "trained guide per workload, shorter reference renders), on (three guides, full "
+                         "references), off, or only (print just that, full)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures a single MI355X; --gpus must be 1 (the multi-GPU split is a "
@@ -686,8 +688,27 @@ def main():
         # against one core of the same CPU baseline (its render-loop rate per process)
         "gpu_over_cpu_core": (value / cpu["value_per_core"]) if cpu else None,
     }
-    if args.guided == "on":
-        out["guided"] = {k: guided_comparison(k, **v) for k, v in GUIDED_WORKLOADS.items()}
+    if args.guided in ("on", "quick"):
+        quick = args.guided == "quick"
+        t_g = time.perf_counter()
+        res = {}
+        for k, v in GUIDED_WORKLOADS.items():
+            kw = dict(v)
+            if quick:
+                kw["repeats"] = 1
+                kw["ref_spp"] = kw["ref_spp"] // 4
+            res[k] = guided_comparison(k, **kw)
+        # the headline half of the metric: time to the plain tracer's relMSE, guided vs plain, on the
+        # 1024-bin media streak film and on the icosphere geometry in the medium
+        out["guided"] = {
+            "metric": "seconds to the plain tracer's relMSE after 64 passes, plain / guided (trimmed per-bin relMSE)",
+            "mode": args.guided, "seconds": time.perf_counter() - t_g,
+            "speedup": {k: r["speedup"] for k, r in res.items()},
+            "speedup_by_metric": {k: r["speedup_by_metric"] for k, r in res.items()},
+            "error_ratio_at_equal_spp": {k: r["error_ratio_at_equal_spp"] for k, r in res.items()},
+            "target": ">= 2x (guided reaches plain's relMSE in <= 0.5x the wall time)",
+            "detail": res,
+        }
     print(json.dumps(out))
 
 

@@ -160,9 +160,9 @@ typedef struct {
  * compensated FP32 sum, two arrays of rows x W x T x 3 floats (rows = 1 in scanline mode): the
  * running sum and the exact rounding errors of the additions into it. Every addition into an
  * element (each sample of a render, each LDS partial the film kernel flushes, each caller sample)
- * is compensated, so sum + comp is the exact total up to second-order terms (about n * 2^-48 of
- * the total after n additions, against n * 2^-24 for a plain FP32 running sum). A third array of
- * the same size is allocated by the first tr_film_read for the normalised copy. */
+ * is compensated, so sum + comp is within (2u + O(n u^2)) sum|x| of the exact total after n
+ * additions (u = 2^-24), against n u sum|x| for a plain FP32 running sum, whatever the order of the
+ * additions. A staging array for the normalised copy is allocated by the first read. */
 int tr_film_create(tr_film* out, const tr_film_desc* desc);
 int tr_film_destroy(tr_film film);
 /* Film::clean: zero all sums and weights. */
@@ -238,8 +238,9 @@ enum {
      * reference's frame (Intersection.h:167-178 builds a tangent that is not orthogonal to n
      * when n.z != 0); guided renders always use the orthonormal frame */
     TR_RENDER_ORTHONORMAL_FRAME = 4u,
-    /* capture the kernels of one wavefront (camera + every bounce) in a hipGraph once per call
-     * and replay it for each wavefront; per-kernel timing is not available in this mode */
+    /* capture the whole render — every wavefront's camera and bounce kernels — in one hipGraph
+     * and submit it with a single launch (guided renders: one wavefront's kernels, replayed per
+     * wavefront around the training steps); per-kernel timing is not available in this mode */
     TR_RENDER_GRAPH = 8u,
     /* the integrator's Transient sampling mode, -time-sampling (BidirectionalPathTracing.h:190-198,
      * bunnykiller.cpp:1811-1817): free flights at rate 4 / (T * exposure), directions at medium
@@ -321,10 +322,15 @@ int tr_batch_clear(tr_batch batch);
  * stats aggregates all jobs. */
 int tr_batch_render(tr_batch batch, const tr_launch_desc* launch, tr_render_stats* stats);
 
-/* Autotuner: renders the batch's jobs at one camera sample per pixel under each candidate launch
- * shape (wavefront size x traversal blocks per CU x shading blocks per CU, with and without
- * graph replay), keeps the fastest for later tr_batch_render calls and returns it in `best`
- * (optional). The films are cleared afterwards. `ms_best` (optional) receives its time. */
+/* Autotuner: times candidate launch shapes on the batch's own jobs and keeps the fastest for later
+ * tr_batch_render calls (returned in `best`, optional; its trial time in `ms_best`, optional).
+ * Trials render every job with sqrt_spp capped at the smallest value for which the capped batch
+ * holds min(all paths, 2^25) paths (about one largest-candidate wavefront), best of two runs each:
+ * stage 1 wavefront sizes 2^18..2^25 with and without graph replay, stage 2 traversal / shading
+ * blocks per CU, stage 4 the traversal formats (32-bit stack, keyed stack, postponed leaves).
+ * Stage 3 renders the FULL jobs once in each batch mode (shared wavefronts; TR_RENDER_PER_JOB), so
+ * the call costs at least two complete batch renders plus the trials. A candidate replaces the
+ * incumbent only when more than 2 % faster. Every film is cleared afterwards. */
 int tr_autotune(tr_batch batch, tr_launch_desc* best, double* ms_best);
 
 /* ---- path guiding (no counterpart in the reference) ---------------------------------------

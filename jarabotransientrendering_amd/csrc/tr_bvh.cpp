@@ -5,6 +5,7 @@
 #include "tr_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -306,14 +307,186 @@ int32_t collapse(const std::vector<BvhNodeHost>& bin, int32_t bnode, std::vector
 
 }  // namespace
 
+// ---- SAH-optimal collapse (dynamic programming over the binary tree) ---------------------------
+// Which binary nodes become 4-wide nodes is chosen to minimise the surface-area cost
+//   sum over wide nodes of area * kCostNode + sum over leaves of area * count * kCostTri
+// (the greedy collapse above opens the largest child instead). Ylitie, Karras, Laine, "Efficient
+// Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017, sec. 4.1: for a binary
+// node x and j slots, D(x, j) is the least cost of x's subtree placed in j slots of its parent's
+// wide node: D(x, 1) = x as a wide node of its own (area(x) * kCostNode + its best 2..4-slot
+// distribution), D(x, j > 1) = the best split of the j slots between x's two children. The binary
+// builder's leaves (at most kLeafMax triangles) are kept as they are.
+namespace {
+
+constexpr float kCostNode = 1.0f;  // a node step of the traversal
+constexpr float kCostTri = 0.4f;   // one triangle test, relative to a node step (instruction counts)
+
+struct SahCollapse {
+    const std::vector<BvhNodeHost>& bin;
+    // per binary inner node: D[j-1] for j = 1..4 and the split of D(x, j): slots of the left child
+    std::vector<std::array<float, 4>> D;
+    std::vector<std::array<int8_t, 4>> split;
+
+    explicit SahCollapse(const std::vector<BvhNodeHost>& b) : bin(b), D(b.size()), split(b.size())
+    {
+        for (auto& d : D) d.fill(-1.0f);
+    }
+
+    // cost of child reference c given k >= 1 slots (leaves and empty slots take one)
+    float slot_cost(const ChildRef& c, int k)
+    {
+        if (c.count < 0) return 0.0f;
+        if (c.count > 0) return c.area() * kCostTri * float(c.count);
+        float best = std::numeric_limits<float>::infinity();
+        for (int m = 1; m <= k; ++m) best = std::min(best, dist(c.index, m));
+        return best;
+    }
+    // best number of slots m <= k for child c (0 for leaves / empty)
+    int slot_choice(const ChildRef& c, int k)
+    {
+        if (c.count != 0) return 1;
+        int bm = 1;
+        float best = std::numeric_limits<float>::infinity();
+        for (int m = 1; m <= k; ++m)
+            if (dist(c.index, m) < best) {
+                best = dist(c.index, m);
+                bm = m;
+            }
+        return bm;
+    }
+    float dist(int32_t x, int j)
+    {
+        float& memo = D[size_t(x)][size_t(j - 1)];
+        if (memo >= 0.0f) return memo;
+        const ChildRef l = child_of(bin[size_t(x)], 0), r = child_of(bin[size_t(x)], 1);
+        if (j == 1) {
+            // x as a wide node of its own: its box plus its children in 2..4 slots
+            float best = std::numeric_limits<float>::infinity();
+            for (int m = 2; m <= 4; ++m) best = std::min(best, dist(x, m));
+            const float dx = bin_area(x);
+            memo = dx * kCostNode + best;
+            split[size_t(x)][0] = 0;
+            return memo;
+        }
+        float best = std::numeric_limits<float>::infinity();
+        int8_t bs = 1;
+        for (int a = 1; a < j; ++a) {
+            const float c = slot_cost(l, a) + slot_cost(r, j - a);
+            if (c < best) {
+                best = c;
+                bs = int8_t(a);
+            }
+        }
+        memo = best;
+        split[size_t(x)][size_t(j - 1)] = bs;
+        return memo;
+    }
+    float bin_area(int32_t x) const
+    {
+        ChildRef l = child_of(bin[size_t(x)], 0), r = child_of(bin[size_t(x)], 1);
+        ChildRef u = l;
+        for (int a = 0; a < 3; ++a) {
+            if (r.count >= 0) {
+                u.lo[a] = l.count >= 0 ? std::min(l.lo[a], r.lo[a]) : r.lo[a];
+                u.hi[a] = l.count >= 0 ? std::max(l.hi[a], r.hi[a]) : r.hi[a];
+            }
+        }
+        return u.area();
+    }
+    // the child references of x's subtree placed in j slots (x opened), appended to `out`
+    void gather(int32_t x, int j, std::vector<ChildRef>& out)
+    {
+        const ChildRef l = child_of(bin[size_t(x)], 0), r = child_of(bin[size_t(x)], 1);
+        const int a = split[size_t(x)][size_t(j - 1)];
+        place(l, a, out);
+        place(r, j - a, out);
+    }
+    void place(const ChildRef& c, int k, std::vector<ChildRef>& out)
+    {
+        if (c.count != 0) {
+            out.push_back(c);
+            return;
+        }
+        const int m = slot_choice(c, k);
+        if (m == 1) out.push_back(c);  // its own wide node
+        else gather(c.index, m, out);
+    }
+    int32_t emit(int32_t x, std::vector<Bvh4NodeHost>& out, int depth, int* max_depth)
+    {
+        int bm = 2;
+        float best = std::numeric_limits<float>::infinity();
+        for (int m = 2; m <= 4; ++m)
+            if (dist(x, m) < best) {
+                best = dist(x, m);
+                bm = m;
+            }
+        std::vector<ChildRef> refs;
+        gather(x, bm, refs);
+        const int32_t me = int32_t(out.size());
+        out.emplace_back();
+        *max_depth = std::max(*max_depth, depth);
+        const float inf = std::numeric_limits<float>::infinity();
+        for (int k = 0; k < 4; ++k) {
+            const bool used = k < int(refs.size()) && refs[size_t(k)].count >= 0;
+            const ChildRef* c = used ? &refs[size_t(k)] : nullptr;
+            out[size_t(me)].lox[k] = used ? c->lo[0] : inf;
+            out[size_t(me)].hix[k] = used ? c->hi[0] : -inf;
+            out[size_t(me)].loy[k] = used ? c->lo[1] : inf;
+            out[size_t(me)].hiy[k] = used ? c->hi[1] : -inf;
+            out[size_t(me)].loz[k] = used ? c->lo[2] : inf;
+            out[size_t(me)].hiz[k] = used ? c->hi[2] : -inf;
+            out[size_t(me)].child[k] = used ? c->index : 0;
+            out[size_t(me)].count[k] = used ? c->count : -1;
+        }
+        for (int k = 0; k < int(refs.size()); ++k)
+            if (refs[size_t(k)].count == 0) {
+                const int32_t c = emit(refs[size_t(k)].index, out, depth + 1, max_depth);
+                out[size_t(me)].child[k] = c;  // `out` may have grown: index, do not hold references
+            }
+        return me;
+    }
+};
+
+// Entries a traversal stack can hold at most: a node step pushes its entered inner children but the
+// one it descends into (or all of them when it also has entered leaves, for any-hit queries), so
+// along any root-to-node path at most sum(inner children - 1, or inner children next to a leaf)
+// entries wait; the largest such sum over the tree.
+int stack_bound(const std::vector<Bvh4NodeHost>& nodes, int32_t n)
+{
+    int inner = 0, best = 0;
+    bool leaf = false;
+    for (int k = 0; k < 4; ++k) {
+        if (nodes[size_t(n)].count[k] == 0) ++inner;
+        if (nodes[size_t(n)].count[k] > 0) leaf = true;
+    }
+    for (int k = 0; k < 4; ++k)
+        if (nodes[size_t(n)].count[k] == 0) best = std::max(best, stack_bound(nodes, nodes[size_t(n)].child[k]));
+    return best + (leaf ? inner : std::max(inner - 1, 0));
+}
+
+}  // namespace
+
 Bvh4 collapse_bvh4(const BvhBuildResult& b)
+{
+    Bvh4 r;
+    r.nodes.reserve(b.nodes.size() / 2 + 2);
+    int depth = 0;
+    SahCollapse sc(b.nodes);
+    sc.emit(0, r.nodes, 1, &depth);
+    r.depth = depth;
+    r.stack_bound = stack_bound(r.nodes, 0);
+    return r;
+}
+
+// The greedy collapse (open the child of largest area until four children), kept for comparison.
+Bvh4 collapse_bvh4_greedy(const BvhBuildResult& b)
 {
     Bvh4 r;
     r.nodes.reserve(b.nodes.size() / 2 + 2);
     int depth = 0;
     collapse(b.nodes, 0, r.nodes, 1, &depth);
     r.depth = depth;
-    r.stack_bound = 3 * depth;
+    r.stack_bound = stack_bound(r.nodes, 0);
     return r;
 }
 

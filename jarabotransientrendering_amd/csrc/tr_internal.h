@@ -71,9 +71,11 @@ struct Bvh4 {
     int stack_bound = 0;               // entries a traversal stack may hold (3 per level)
 };
 
-// Collapses a binary BVH: every 4-wide node takes the two children of a binary node and opens the
-// inner child of largest surface area until it has four children or only leaves remain.
+// Collapses a binary BVH into a 4-wide one. collapse_bvh4 picks the binary nodes that become wide
+// nodes by minimising the SAH cost (dynamic programming); collapse_bvh4_greedy opens the inner child
+// of largest surface area until a node has four children or only leaves remain.
 Bvh4 collapse_bvh4(const BvhBuildResult& b);
+Bvh4 collapse_bvh4_greedy(const BvhBuildResult& b);
 
 // ---- quantized 4-wide node: 64 bytes (four 16-byte words) --------------------------------------
 //   word 0: origin xyz (float), w = grid step of z (a power of two, as a float)

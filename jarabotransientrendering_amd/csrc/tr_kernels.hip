@@ -1028,10 +1028,10 @@ __global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, con
 // film accumulation with compensated sums. Every film element is a pair of FP32 accumulators
 // (sum, comp). An addition of x is an atomic add that returns the old value a; the rounding error
 // of the stored a + x is then recovered exactly with Knuth's TwoSum and added to comp, so after
-// any number of additions in any order sum + comp equals the exact total up to the rounding of the
-// additions into comp (errors of the errors: second order, about n * eps^2 of the total instead of
-// the n * eps of a plain FP32 running sum). The hardware atomic rounds to nearest-even like the
-// vector ALU, which is what makes the recovered error exact.
+// any number n of additions in any order sum + comp equals the exact total up to the rounding of
+// the additions into comp: within (2u + O(n u^2)) sum|x| (u = 2^-24), against n u sum|x| for a plain
+// FP32 running sum. The hardware atomic rounds to nearest-even like the vector ALU, which is what
+// makes the recovered error exact.
 __device__ __forceinline__ float two_sum_err(float a, float b, float s)
 {
     // s = fl(a + b); returns (a + b) - s exactly
@@ -2621,7 +2621,34 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
         graph = nullptr;
     };
     uint32_t n_waves = 0;
-    for (uint64_t base = 0; base < total; base += c.wave) {
+    if (c.graph && !guide) {
+        // the whole render (every wavefront's set-up, camera and bounce kernels) as one captured
+        // graph, submitted with one launch; guided renders instead replay one wavefront's kernels
+        // (their Adam steps between wavefronts take host-computed bias corrections)
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (uint64_t base = 0; base < total && !rc; base += c.wave) {
+            const uint32_t n = uint32_t(std::min<uint64_t>(c.wave, total - base));
+            ++n_waves;
+            set_wave_kernel<<<1, 1, 0, stream>>>(wb.ctl, base, n);
+            rc = L.enqueue_wave();
+        }
+        const hipError_t ce = hipStreamEndCapture(stream, &graph);
+        if (rc || ce != hipSuccess) {
+            drop_graph();
+            return rc ? rc : fail(TR_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            drop_graph();
+            return fail(TR_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        }
+        const hipError_t le = hipGraphLaunch(exec, stream);
+        if (le != hipSuccess) {
+            drop_graph();
+            return fail(TR_ERR_DEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(le));
+        }
+    }
+    for (uint64_t base = 0; base < total && !(c.graph && !guide); base += c.wave) {
         uint32_t n = uint32_t(std::min<uint64_t>(c.wave, total - base));
         ++n_waves;
         if (guide) {
