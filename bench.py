@@ -342,6 +342,11 @@ def cpu_baseline_batch(n_jobs: int, seconds: float):
             "value_per_core": per_core, "host": host, "calibration": CPU_CALIBRATION}
 
 
+def progress(msg: str) -> None:
+    """A progress line on stderr (the JSON result stays the only line on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def bench_batch(args) -> dict:
     n_jobs = 64
     cpu = None if args.no_cpu_baseline else cpu_baseline_batch(n_jobs, args.cpu_seconds)
@@ -350,11 +355,13 @@ def bench_batch(args) -> dict:
     scs = scenes.batch_scenes(n_jobs)
     setups = [renderer.setup_scene(s) for s in scs]
     build_s = time.perf_counter() - t0
+    progress(f"batch: {n_jobs} scenes built in {build_s:.1f} s")
     b = renderer.Batch()
     for (w, cam, sampler, film, integ), s in zip(setups, scs):
         b.add(w, integ, cam, film, sampler, seed=s.render.seed)
     tuned, tune_ms = b.autotune()
     launch = dict(tuned)
+    progress(f"batch: autotuned {launch}")
     for k in range(args.warmup):
         b.render(launch)
     for (_, _, _, film, _) in setups:
@@ -383,7 +390,9 @@ def bench_batch(args) -> dict:
     t3 = time.perf_counter()
     other_paths = b.render(other)["paths"]
     other_rate = other_paths / (time.perf_counter() - t3)
+    progress("batch: timed and profiled renders done")
     small = small_jobs_comparison(renderer)
+    progress("batch: small-jobs comparison done")
     # the same jobs one render call at a time
     eng = renderer.TransientRenderer()
     t2 = time.perf_counter()
@@ -595,6 +604,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.workload, args.cpu_seconds)
+        progress(f"cpu baseline: {cpu['value']:.4g} paths/s on {cpu['cores']} processes")
 
     from jarabotransientrendering_amd import renderer, _native as nat
     world, cam, sampler, film, integ = renderer.setup_scene(scene)
@@ -693,6 +703,7 @@ def main():
         t_g = time.perf_counter()
         res = {}
         for k, v in GUIDED_WORKLOADS.items():
+            progress(f"guided comparison: {k}")
             kw = dict(v)
             if quick:
                 kw["repeats"] = 1

@@ -159,8 +159,8 @@ typedef struct {
 /* StreakCameraFilm::StreakCameraFilm (src/Film/StreakCameraFilm.h:91-121). Device buffers: a
  * compensated FP32 sum, two arrays of rows x W x T x 3 floats (rows = 1 in scanline mode): the
  * running sum and the exact rounding errors of the additions into it. Every addition into an
- * element (each sample of a render, each LDS partial the film kernel flushes, each caller sample)
- * is compensated, so sum + comp is within (2u + O(n u^2)) sum|x| of the exact total after n
+ * element (each sample of a render, each FP64 LDS partial the film kernel flushes, each caller
+ * sample) is compensated, so sum + comp is within (2u + O(n u^2)) sum|x| of the exact total after n
  * additions (u = 2^-24), against n u sum|x| for a plain FP32 running sum, whatever the order of the
  * additions. A staging array for the normalised copy is allocated by the first read. */
 int tr_film_create(tr_film* out, const tr_film_desc* desc);

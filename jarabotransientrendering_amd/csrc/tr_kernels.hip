@@ -1058,37 +1058,31 @@ __device__ __forceinline__ void film_add_partial(float* __restrict__ sum, float*
     if (c != 0.0f) unsafeAtomicAdd(comp + i, c);
 }
 
-// x into the compensated pair (hs[i], hc[i]) of an LDS histogram
-__device__ __forceinline__ void lds_add(float* hs, float* hc, uint32_t i, float x)
-{
-    const float a = atomicAdd(hs + i, x);
-    const float e = two_sum_err(a, x, a + x);
-    if (e != 0.0f) atomicAdd(hc + i, e);
-}
-
 // film: bins the visible contributions of this bounce into the film. A block takes a range of
 // kFilmRange consecutive shadow records; their pixels are few (paths are enumerated pixel-major
 // and queue compaction keeps runs of neighbouring paths together), so each block accumulates into
-// compensated LDS histograms of up to `pix_slots` pixels x T bins x RGB and flushes each non-zero
-// bin's (sum, comp) partial with one compensated add; records of further pixels go straight to
-// HBM. Short ranges (512 records, two 1024-bin pixels) measured best: 1.6x faster than 4096-record
-// ranges on the 1024-bin films, whose per-pixel records spread over many bins.
+// LDS histograms of up to `pix_slots` pixels x T bins x RGB and flushes each non-zero bin with one
+// compensated add; records of further pixels go straight to HBM. The LDS histograms hold FP64
+// partials (non-returning LDS atomic adds): a partial of at most kFilmRange float contributions is
+// exact to within 2^-44 of their magnitude sum, far inside the film's compensated FP32 bound, and
+// is flushed as the float pair (hi, lo) = (float(p), float(p - hi)). Short ranges (512 records,
+// two 1024-bin pixels) measured best: 1.6x faster than 4096-record ranges on the 1024-bin films,
+// whose per-pixel records spread over many bins.
 // (StreakCameraFilm::add_samples / add_temporal_sample, StreakCameraFilm.h:139-192, 280-297)
 constexpr int kFilmBlock = 256;
 constexpr uint32_t kFilmRange = 512;
-constexpr uint32_t kFilmLdsBytes = 24 * 1024;  // per accumulator half (sums; as much again for comp)
+constexpr uint32_t kFilmLdsBytes = 48 * 1024;  // FP64 histograms: pix_slots * T * 3 * 8 bytes
 constexpr uint32_t kFilmMaxSlots = 8;
 
 template <bool GUIDED>
 __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev Wv, uint32_t pix_slots, float* contrib)
 {
-    extern __shared__ float hist[];          // sums: pix_slots * T * 3, then as many compensations
+    extern __shared__ double hist[];         // pix_slots * T * 3 partials
     __shared__ uint32_t slot_pix[kFilmMaxSlots];
     const uint32_t count = Wv.counters[C_SHADOW];
     const uint32_t T = uint32_t(R.T), T3 = 3u * T, hsize = pix_slots * T3;
-    float* hcomp = hist + hsize;
     // the histograms start at zero; each flush reads and clears only the slots that were used
-    for (uint32_t i = threadIdx.x; i < 2u * hsize; i += blockDim.x) hist[i] = 0.0f;
+    for (uint32_t i = threadIdx.x; i < hsize; i += blockDim.x) hist[i] = 0.0;
     for (uint32_t start = blockIdx.x * kFilmRange; start < count; start += gridDim.x * kFilmRange) {
         const uint32_t end = min(start + kFilmRange, count);
         if (threadIdx.x < kFilmMaxSlots) slot_pix[threadIdx.x] = 0xFFFFFFFFu;
@@ -1112,10 +1106,10 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
                 }
             }
             if (slot >= 0) {
-                const uint32_t h = uint32_t(slot) * T3 + bin * 3u;
-                lds_add(hist, hcomp, h + 0, c.x);
-                lds_add(hist, hcomp, h + 1, c.y);
-                lds_add(hist, hcomp, h + 2, c.z);
+                double* h = hist + uint32_t(slot) * T3 + bin * 3u;
+                atomicAdd(h + 0, double(c.x));
+                atomicAdd(h + 1, double(c.y));
+                atomicAdd(h + 2, double(c.z));
             } else {
                 const size_t e = (size_t(pix) * T + bin) * 3;
                 film_add(R.film, R.film_comp, e + 0, c.x);
@@ -1127,12 +1121,12 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
         uint32_t used = 0;  // slots are claimed in order
         while (used < pix_slots && slot_pix[used] != 0xFFFFFFFFu) ++used;
         for (uint32_t i = threadIdx.x; i < used * T3; i += blockDim.x) {
-            const float v = hist[i], vc = hcomp[i];
-            if (v != 0.0f || vc != 0.0f) {
+            const double v = hist[i];
+            if (v != 0.0) {
                 const uint32_t k = i / T3;
-                film_add_partial(R.film, R.film_comp, size_t(slot_pix[k]) * T3 + (i - k * T3), v, vc);
-                hist[i] = 0.0f;
-                hcomp[i] = 0.0f;
+                const float hi = float(v), lo = float(v - double(hi));
+                film_add_partial(R.film, R.film_comp, size_t(slot_pix[k]) * T3 + (i - k * T3), hi, lo);
+                hist[i] = 0.0;
             }
         }
         __syncthreads();
@@ -2256,8 +2250,8 @@ int tr_film_add_samples(tr_film f, const tr_film_sample* samples, uint32_t n)
     Wv.sh_d = dd;
     Wv.sh_vis = dv;
     Wv.counters = dcount;
-    const uint32_t pix_slots = std::min<uint32_t>(kFilmMaxSlots, kFilmLdsBytes / (T * 12u));
-    const size_t lds = 2u * size_t(pix_slots) * size_t(T) * 12u;
+    const uint32_t pix_slots = std::min<uint32_t>(kFilmMaxSlots, kFilmLdsBytes / (T * 24u));
+    const size_t lds = size_t(pix_slots) * size_t(T) * 24u;
     const uint32_t fg = std::max(1u, std::min((n + kFilmRange - 1) / kFilmRange, uint32_t(tri::compute_units()) * 4u));
     hipError_t e = hipMemcpyAsync(dc, hc.data(), size_t(n) * sizeof(float4), hipMemcpyHostToDevice, f->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dd, hd.data(), size_t(n) * sizeof(float4), hipMemcpyHostToDevice, f->stream);
@@ -2601,12 +2595,12 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
         L.events_on = true;
         L.extend_only = c.prof_extend;
     }
-    // LDS histogram slots of the film kernel: as many pixels x T bins x RGB as fit in kFilmLdsBytes,
-    // with a compensation beside each sum
+    // LDS histogram slots of the film kernel: as many pixels x T bins x RGB FP64 partials as fit in
+    // kFilmLdsBytes
     {
         const uint32_t T = uint32_t(jobs[0].dev.R.T);
-        L.pix_slots = std::min<uint32_t>(kFilmMaxSlots, kFilmLdsBytes / (T * 12u));
-        L.film_lds = 2u * size_t(L.pix_slots) * size_t(T) * 12u;
+        L.pix_slots = std::min<uint32_t>(kFilmMaxSlots, kFilmLdsBytes / (T * 24u));
+        L.film_lds = size_t(L.pix_slots) * size_t(T) * 24u;
     }
     HIP_TRY(hipMemsetAsync(wb.counters, 0, C_WORDS * sizeof(uint32_t), stream));
     // pass buffers are written on this stream; order it after any pending film work
