@@ -132,6 +132,16 @@ int tr_intersect_batch(tr_scene scene, const tr_ray* rays, uint32_t n, tr_hit* h
 /* Same, on device pointers, asynchronously on `stream` (a hipStream_t; NULL = library stream). */
 int tr_intersect_batch_device(tr_scene scene, const tr_ray* d_rays, uint32_t n, tr_hit* d_hits,
                               void* stream);
+/* tr_intersect_batch without allocating device memory. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). Rays and hits are staged in a static device array of TR_NOALLOC_MAX_RAYS x 48 bytes
+ * (3 MiB) on each GPU the library runs on, held until the process ends; n above
+ * TR_NOALLOC_MAX_RAYS fails with TR_ERR_INVALID. Every call on one GPU shares that array: calls
+ * that overlap on one GPU (from several threads) race and must be serialised by the caller.
+ * Host pointers; same results as tr_intersect_batch. */
+#define TR_NOALLOC_MAX_RAYS 65536u
+int tr_intersect_batch_noalloc(tr_scene scene, const tr_ray* rays, uint32_t n, tr_hit* hits);
+
 /* Any hit in (tnear, tfar] for each ray: occluded[i] = 1 if some triangle is hit there, else 0 —
  * World::intersects (src/RayTracing/World.h:617-639 -> rtcIntersect, geomID test) and
  * World::is_visible (World.h:641-664 -> rtcOccluded; visible = !occluded for the segment

@@ -19,6 +19,7 @@ TR_ERR_NOMEM = -3
 TR_ERR_STATE = -4
 
 TR_MATERIAL_TWO_SIDED = 1
+TR_NOALLOC_MAX_RAYS = 65536  # rays per tr_intersect_batch_noalloc call
 TR_RENDER_PROFILE = 1
 TR_RENDER_PROFILE_EXTEND = 256
 TR_RENDER_BINARY_BVH = 2
@@ -150,6 +151,8 @@ SIGNATURES = {
     "tr_intersect_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
     "tr_intersect_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                  ctypes.c_void_p, ctypes.c_void_p]),
+    "tr_intersect_batch_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                  ctypes.c_void_p]),
     "tr_occluded_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
     "tr_occluded_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                 ctypes.c_void_p, ctypes.c_void_p]),

@@ -1931,6 +1931,71 @@ int tr_intersect_batch(tr_scene s, const tr_ray* rays, uint32_t n, tr_hit* hits)
     return out;
 }
 
+}  // extern "C"
+
+// ---- closest-hit queries without allocation ---------------------------------------------------
+// Staging for tr_intersect_batch_noalloc: rays, then hits, of at most TR_NOALLOC_MAX_RAYS rays, in
+// one static device array per GPU (held until the process ends). Host code never passes the
+// array's name to a kernel or a copy: it takes the device address from hipGetSymbolAddress once
+// per device and uses that.
+namespace {
+constexpr size_t kNoAllocRayBytes = size_t(TR_NOALLOC_MAX_RAYS) * sizeof(tr_ray);
+constexpr size_t kNoAllocBytes = kNoAllocRayBytes + size_t(TR_NOALLOC_MAX_RAYS) * sizeof(tr_hit);
+static_assert(kNoAllocRayBytes % 256 == 0, "hits start 256-byte aligned");
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of this device's scratch array (looked up on the first call on each device)
+int scratch_on_current_device(void** out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(TR_ERR_DEVICE, "tr_intersect_batch_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(TR_ERR_DEVICE, std::string("tr_intersect_batch_noalloc: hipGetSymbolAddress: ") +
+                                           hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TR_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int tr_intersect_batch_noalloc(tr_scene s, const tr_ray* rays, uint32_t n, tr_hit* hits)
+{
+    if (!s) return fail(TR_ERR_INVALID, "tr_intersect_batch_noalloc: null scene");
+    if (!s->committed) return fail(TR_ERR_STATE, "tr_intersect_batch_noalloc: scene not committed");
+    if (n == 0) return TR_OK;
+    if (!rays || !hits) return fail(TR_ERR_INVALID, "tr_intersect_batch_noalloc: null buffer");
+    if (n > TR_NOALLOC_MAX_RAYS)
+        return fail(TR_ERR_INVALID, "tr_intersect_batch_noalloc: " + std::to_string(n) + " rays exceed the limit of " +
+                                        std::to_string(TR_NOALLOC_MAX_RAYS) + " per call");
+    void* base = nullptr;
+    int rc = scratch_on_current_device(&base);
+    if (rc) return rc;
+    tr_ray* dr = static_cast<tr_ray*>(base);
+    tr_hit* dh = reinterpret_cast<tr_hit*>(static_cast<unsigned char*>(base) + kNoAllocRayBytes);
+    int out = TR_OK;
+    if (hipMemcpyAsync(dr, rays, size_t(n) * sizeof(tr_ray), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        out = fail(TR_ERR_DEVICE, "tr_intersect_batch_noalloc: upload failed");
+    if (!out) out = tr_intersect_batch_device(s, dr, n, dh, s->stream);
+    if (!out && hipMemcpyAsync(hits, dh, size_t(n) * sizeof(tr_hit), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+        out = fail(TR_ERR_DEVICE, "tr_intersect_batch_noalloc: download failed");
+    // also on failure: the queued copies read and write the caller's buffers
+    if (hipStreamSynchronize(s->stream) != hipSuccess && !out)
+        out = fail(TR_ERR_DEVICE, "tr_intersect_batch_noalloc: kernel failed");
+    return out;
+}
+
 int tr_occluded_batch_device(tr_scene s, const tr_ray* d_rays, uint32_t n, uint8_t* d_occ, void* stream)
 {
     if (!s) return fail(TR_ERR_INVALID, "tr_occluded_batch_device: null scene");
