@@ -653,12 +653,19 @@ def main():
     bytes_per_launch = alg_bytes[dom] / max(dom_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
+    limiter = None
     try:
         with open(args.traffic_json) as fh:
             tj = json.load(fh)
         key = f"{args.workload}:{dom}"
         if key in tj:
             traffic = tj[key]["bytes_per_launch"]
+            # what bounds the kernel, from the same profile's SQ counters: the share of SIMD cycles
+            # that issue a VALU instruction, and the active lanes per instruction (of 64)
+            if "valu_issue_frac" in tj[key]:
+                limiter = {"kind": "valu_issue", "valu_issue_frac": round(tj[key]["valu_issue_frac"], 3),
+                           "active_lanes_per_valu": round(tj[key]["active_lanes_per_valu"], 1), "lanes": 64,
+                           "source": "profiles/icosphere6_valu_counters.md"}
     except (OSError, ValueError, KeyError):
         traffic = None
     gpu_time_ms = sum(kern_ms.values())
@@ -682,7 +689,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": avg_ms, "launches": dom_n,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch, "limiter": limiter},
         "path_roofline": {"bytes_per_segment": BYTES_PATH_PER_SEGMENT, "segments_per_path": ext / paths,
                           "achieved": path_gbs, "frac": path_gbs / HBM_PEAK_GBS},
         # per-kernel split from the same number of further steps with every kernel timed
