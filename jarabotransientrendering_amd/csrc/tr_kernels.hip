@@ -68,13 +68,21 @@ constexpr int32_t kMatIndex = kMatSmooth - 1;
 // multibounce films: bits of a shadow record's tag that hold its layer (at most 32 layers)
 constexpr uint32_t kMaxLayersTagBits = kMbLayerTagBits;
 
-// counters[] layout (uint32 words)
+// counters[] layout (uint32 words): queue lengths and per-launch statistics in the first 256 bytes,
+// then kParts partitions of 256 bytes each holding the work-fetch cursors of the persistent
+// traversal grids and the visible-contribution count. Atomics to one address are serialised, and
+// the wavefronts of a launch fetch work all the time: blocks are dealt to the 8 XCDs in turn, so
+// partition blockIdx.x % kParts takes the fetches of one XCD's blocks on its own 256-byte line.
+constexpr uint32_t kParts = 8;
+constexpr uint32_t kPartBase = 64;
+constexpr uint32_t kPartWords = 64;
 enum {
-    C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2,   // queue lengths
-    C_STAT_EXT = 4, C_STAT_SHADOW = 6, C_STAT_CONTRIB = 8,  // 64-bit statistics
-    C_FETCH_EXT = 10, C_FETCH_SHADOW = 11,              // work-fetch cursors of the persistent traversal grids
-    C_WORDS = 16
+    C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2,                    // queue lengths
+    C_STAT_EXT = 4, C_STAT_SHADOW = 6,                   // 64-bit statistics
+    C_WORDS = kPartBase + kParts * kPartWords
 };
+enum { P_FETCH_EXT = 0, P_FETCH_SHADOW = 1, P_STAT_CONTRIB = 2 };  // words of a partition (P_STAT_CONTRIB: 64-bit)
+__host__ __device__ constexpr uint32_t part_word(uint32_t part, uint32_t w) { return kPartBase + part * kPartWords + w; }
 
 struct SceneDev {
     const float4* nodes;       // binary BVH (4 float4 per node)
@@ -187,10 +195,8 @@ __global__ void __launch_bounds__(kBlock) camera_kernel(SceneDev S0, RenderDev R
     const WaveCtl ctl = *Wv.ctl;
     const uint32_t n = ctl.n;
     uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot == 0) {
-        Wv.counters[C_Q0] = n;
-        Wv.counters[C_FETCH_EXT] = 0;
-    }
+    if (slot == 0) Wv.counters[C_Q0] = n;
+    if (slot < kParts) Wv.counters[part_word(slot, P_FETCH_EXT)] = 0;
     if (slot >= n) return;
     // a job holds fewer than 2^32 paths: 32-bit index arithmetic within the job
     uint32_t jid = 0, pid_local;
@@ -284,30 +290,47 @@ constexpr int stack_kind() { return TRAV == kTravWide ? kStack32 : TRAV == kTrav
 // claims `chunk` queue entries at a time from the fetch counter and hands them to its idle lanes
 // in lane order. Call with every lane of the wavefront; `busy` marks the lanes with a query.
 constexpr int kRefillMin = 32;  // idle lanes that trigger a refill (unless the whole wavefront is idle)
+// Share of its even split of the queue that each wavefront takes without an atomic, in quarters
+// (0: one fetch chunk). Larger static shares measured slower on the 82k-triangle scene, whose rays'
+// costs vary across the image, and faster on the 22-triangle media scene, whose launches the
+// fetch atomics dominated before the partitioned cursors.
+constexpr uint32_t kStaticQuarters = 0;
 
 struct RayRefill {
     uint32_t pos = 0, end = 0;  // the wavefront's claimed range of queue entries (uniform)
-    uint32_t dyn_base = 0;      // first queue entry handed out through the fetch counter
-    bool drained = false;       // the queue has no entries left (uniform)
+    uint32_t dyn_base = 0;      // first queue entry handed out through the fetch cursors
+    uint32_t parts = 1, part = 0;
+    bool drained = false;       // the queue has no entries left for this wavefront (uniform)
 
-    // Every wavefront starts on its own first chunk (entries [wave * chunk, (wave + 1) * chunk))
-    // without an atomic; later chunks, in queue order after those, come through the fetch counter.
-    // A launch of cheap rays (a handful of chunks per wavefront) otherwise spends its time on one
-    // contended counter; handing chunks out in queue order keeps the wavefronts on neighbouring
-    // rays (and so on the same part of the BVH in cache).
+    // Every wavefront starts on its own contiguous static share of the queue (entries
+    // [wave * share, (wave + 1) * share), at least one chunk) without an atomic. The entries after
+    // all static shares are dealt in chunks, in queue order, through per-partition cursors: chunk
+    // g = k * parts + p is the k-th fetch of partition p = blockIdx.x % parts, so the partitions'
+    // wavefronts work through interleaved chunks (every partition sees the same mix of cheap and
+    // costly rays) while their fetch atomics go to `parts` different lines. A launch with no
+    // entries beyond the static shares takes no atomic at all (the last bounces of a render, whose
+    // few rays otherwise waited on one failing fetch per wavefront of the grid). Handing chunks out
+    // in queue order keeps the wavefronts on neighbouring rays (and so on the same part of the BVH
+    // in cache).
     __device__ __forceinline__ RayRefill(uint32_t count, uint32_t chunk)
     {
         const uint32_t waves = gridDim.x * (blockDim.x >> 6);
         const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-        const uint64_t first = uint64_t(wave) * chunk, all = uint64_t(waves) * chunk;
+        uint32_t share = (count / waves) * kStaticQuarters / 4u;
+        share = share < chunk ? chunk : share - share % chunk;
+        const uint64_t first = uint64_t(wave) * share, all = uint64_t(waves) * share;
         pos = first < count ? uint32_t(first) : count;
-        end = first + chunk < count ? uint32_t(first + chunk) : count;
+        end = first + share < count ? uint32_t(first + share) : count;
         dyn_base = all < count ? uint32_t(all) : count;
+        parts = gridDim.x < kParts ? gridDim.x : kParts;
+        part = blockIdx.x % parts;
     }
 
     // Assigns queue entries to idle lanes; `start(i)` begins the query of entry i on this lane.
+    // `cursor` is the partition word (P_FETCH_EXT / P_FETCH_SHADOW) of the launch's queue.
     template <class Start>
-    __device__ __forceinline__ void run(uint32_t count, uint32_t chunk, uint32_t* fetch, bool& busy, Start start)
+    __device__ __forceinline__ void run(uint32_t count, uint32_t chunk, uint32_t* counters, uint32_t cursor, bool& busy,
+                                        Start start)
     {
         const uint32_t lane = __lane_id();
         const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
@@ -317,15 +340,20 @@ struct RayRefill {
         if (__popcll(need) < kRefillMin && need != ~0ull) return;
         while (need && !drained) {
             if (pos >= end) {
-                uint32_t b = 0;
-                if (lane == 0) b = atomicAdd(fetch, chunk);
-                b = __shfl(b, 0) + dyn_base;
+                if (dyn_base >= count) {  // nothing beyond the static shares
+                    drained = true;
+                    break;
+                }
+                uint32_t k = 0;
+                if (lane == 0) k = atomicAdd(counters + part_word(part, cursor), 1u);
+                k = __shfl(k, 0);
+                const uint64_t b = uint64_t(dyn_base) + (uint64_t(k) * parts + part) * chunk;
                 if (b >= count) {
                     drained = true;
                     break;
                 }
-                pos = b;
-                end = min(b + chunk, count);
+                pos = uint32_t(b);
+                end = uint32_t(min(b + chunk, uint64_t(count)));
             }
             const uint32_t avail = end - pos;
             const uint32_t rank = uint32_t(__popcll(need & below));
@@ -354,9 +382,9 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
     if (gid == 0) {
         Wv.counters[cur ^ 1] = 0;
         Wv.counters[C_SHADOW] = 0;
-        Wv.counters[C_FETCH_SHADOW] = 0;
         atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_EXT), (unsigned long long)count);
     }
+    if (gid < kParts) Wv.counters[part_word(gid, P_FETCH_SHADOW)] = 0;
     const uint32_t chunk = fetch_chunk(count);
     if constexpr (TRAV == kTravWidePP) {
         // persistent grid with per-lane refill; node steps and triangle steps in wavefront-wide phases
@@ -367,7 +395,7 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
         Trav4 st;
         const float4 *nodes = S0.nodes4, *tris = S0.tris;
         for (;;) {
-            rf.run(count, chunk, Wv.counters + C_FETCH_EXT, busy, [&](uint32_t i) {
+            rf.run(count, chunk, Wv.counters, P_FETCH_EXT, busy, [&](uint32_t i) {
                 rec = i;
                 const float4 d = Wv.ray_d[cur][i];
                 float3 c = Wv.cpos;
@@ -407,7 +435,7 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
         Trav4 st;
         const float4 *nodes = S0.nodes4, *tris = S0.tris;
         for (;;) {
-            rf.run(count, chunk, Wv.counters + C_FETCH_EXT, busy, [&](uint32_t i) {
+            rf.run(count, chunk, Wv.counters, P_FETCH_EXT, busy, [&](uint32_t i) {
                 rec = i;
                 const float4 d = Wv.ray_d[cur][i];
                 float3 c = Wv.cpos;
@@ -434,7 +462,7 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
         const uint32_t lane = __lane_id();
         for (;;) {
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(Wv.counters + C_FETCH_EXT, chunk);
+            if (lane == 0) base = atomicAdd(Wv.counters + part_word(0, P_FETCH_EXT), chunk);
             base = __shfl(base, 0);
             if (base >= count) break;
             for (uint32_t j = 0; j < chunk; j += 64) {
@@ -641,7 +669,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
         __syncthreads();
     }
     const uint32_t count = Wv.counters[cur];
-    if (blockIdx.x == 0 && threadIdx.x == 0) Wv.counters[C_FETCH_EXT] = 0;  // for the next extend
+    if (blockIdx.x == 0 && threadIdx.x < kParts) Wv.counters[part_word(threadIdx.x, P_FETCH_EXT)] = 0;  // for the next extend
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
@@ -978,7 +1006,7 @@ __global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, con
         Trav4 st;
         const float4 *nodes = S0.nodes4, *tris = S0.tris;
         for (;;) {
-            rf.run(count, chunk, Wv.counters + C_FETCH_SHADOW, busy, [&](uint32_t i) {
+            rf.run(count, chunk, Wv.counters, P_FETCH_SHADOW, busy, [&](uint32_t i) {
                 rec = i;
                 if constexpr (BATCH) {
                     const SceneDev& S = jobs[Wv.sh_job[i]].S;
@@ -1002,7 +1030,7 @@ __global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, con
     } else {
         for (;;) {
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(Wv.counters + C_FETCH_SHADOW, chunk);
+            if (lane == 0) base = atomicAdd(Wv.counters + part_word(0, P_FETCH_SHADOW), chunk);
             base = __shfl(base, 0);
             if (base >= count) break;
             for (uint32_t j = 0; j < chunk; j += 64) {
@@ -1021,7 +1049,8 @@ __global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, con
     }
     visible = wave_sum(visible);
     if (lane == 0 && visible)
-        atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_CONTRIB), (unsigned long long)visible);
+        atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + part_word(blockIdx.x % kParts, P_STAT_CONTRIB)),
+                  (unsigned long long)visible);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2769,7 +2798,7 @@ int run_jobs(hipStream_t stream, WaveBuffers& wb, std::vector<HostJob>& jobs, co
         auto rd64 = [&](int w) { return uint64_t(counters[w]) | (uint64_t(counters[w + 1]) << 32); };
         stats->extension_rays = rd64(C_STAT_EXT);
         stats->shadow_rays = rd64(C_STAT_SHADOW);
-        stats->contributions = rd64(C_STAT_CONTRIB);
+        for (uint32_t p = 0; p < kParts; ++p) stats->contributions += rd64(int(part_word(p, P_STAT_CONTRIB)));
         stats->waves = n_waves;
         stats->bounces_max = uint32_t(bounces);
         for (int k = 0; k < TR_K_COUNT; ++k) {
