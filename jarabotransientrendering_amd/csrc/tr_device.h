@@ -70,6 +70,45 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
     return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
+// ---- block-aggregated append to two queues with one atomic ----------------------------------
+// As block_append for two queues whose lengths are the low and high halves of one 64-bit counter
+// (`pair`, 8-byte aligned): each thread passes its items for queue A (0 or 1) and queue B (0 or 1)
+// and gets its positions in both; the block issues a single 64-bit atomic. Per-block counts fit
+// 16 bits (blocks of at most 1024 threads), so the two scans run packed in one 32-bit value.
+__device__ __forceinline__ void block_append2(uint32_t* pair, uint32_t cnt_a, uint32_t cnt_b, uint32_t* smem,
+                                              uint32_t* pos_a, uint32_t* pos_b)
+{
+    const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    const uint32_t mine = cnt_a | (cnt_b << 16);
+    uint32_t x = mine;  // inclusive scan within the wavefront, both halves at once
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) smem[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            uint32_t t = smem[w];
+            smem[w] = total;  // exclusive prefix of wavefront totals
+            total += t;
+        }
+        unsigned long long old = 0ull;
+        if (total)
+            old = atomicAdd(reinterpret_cast<unsigned long long*>(pair),
+                            (unsigned long long)(total & 0xFFFFu) | ((unsigned long long)(total >> 16) << 32));
+        smem[nw] = uint32_t(old);
+        smem[nw + 1] = uint32_t(old >> 32);
+    }
+    __syncthreads();
+    const uint32_t pre = smem[wid] + x - mine;
+    *pos_a = smem[nw] + (pre & 0xFFFFu);
+    *pos_b = smem[nw + 1] + (pre >> 16);
+    __syncthreads();  // smem may be reused by the caller's next iteration
+}
+
 // ---- block-aggregated queue append ----------------------------------------------------------
 // Every thread of the block calls this with its number of items (0 or more); the block issues one
 // atomic on `counter` and each thread gets the first queue position of its items. `smem` holds at

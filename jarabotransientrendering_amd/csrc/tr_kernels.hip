@@ -76,11 +76,16 @@ constexpr uint32_t kMaxLayersTagBits = kMbLayerTagBits;
 constexpr uint32_t kParts = 8;
 constexpr uint32_t kPartBase = 64;
 constexpr uint32_t kPartWords = 64;
+// Queue lengths come in two 64-bit pairs, one per record buffer k: (rays of buffer k, shadow
+// rays made from the paths of the other buffer), so that shade appends to both of its output
+// queues with one atomic.
 enum {
-    C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2,                    // queue lengths
+    C_Q0 = 0,                                            // pairs: words 2k (rays), 2k + 1 (shadow rays)
     C_STAT_EXT = 4, C_STAT_SHADOW = 6,                   // 64-bit statistics
     C_WORDS = kPartBase + kParts * kPartWords
 };
+__host__ __device__ constexpr uint32_t qword(int k) { return uint32_t(2 * k); }
+__host__ __device__ constexpr uint32_t shword(int k) { return uint32_t(2 * k + 1); }
 enum { P_FETCH_EXT = 0, P_FETCH_SHADOW = 1, P_STAT_CONTRIB = 2 };  // words of a partition (P_STAT_CONTRIB: 64-bit)
 __host__ __device__ constexpr uint32_t part_word(uint32_t part, uint32_t w) { return kPartBase + part * kPartWords + w; }
 
@@ -377,11 +382,11 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
     __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
     const int cur = cur_first & 1;
     const bool first = (cur_first & 2) != 0;  // camera rays: the origin is the camera's
-    const uint32_t count = Wv.counters[cur];
+    const uint32_t count = Wv.counters[qword(cur)];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
-        Wv.counters[cur ^ 1] = 0;
-        Wv.counters[C_SHADOW] = 0;
+        Wv.counters[qword(cur ^ 1)] = 0;
+        Wv.counters[shword(cur ^ 1)] = 0;
         atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_EXT), (unsigned long long)count);
     }
     if (gid < kParts) Wv.counters[part_word(gid, P_FETCH_SHADOW)] = 0;
@@ -658,7 +663,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
 {
     static_assert(!(BATCH && MODE == kShadeGuided), "guided renders take one job");
     constexpr bool GUIDED = MODE == kShadeGuided;
-    __shared__ uint32_t scan[kShadeBlock / 64 + 1];
+    __shared__ uint32_t scan[kShadeBlock / 64 + 2];
     __shared__ float glds[GUIDED ? (kShadeBlockGuided / 64) * 64 * trg::kRow : 1];
     __shared__ uint4 gfrag[GUIDED ? trg::kFwdFrags * 64 : 1];
     __shared__ float gbias[GUIDED ? trg::kBiasPad : 1];
@@ -668,7 +673,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
         for (uint32_t k = threadIdx.x; k < uint32_t(trg::kBiasPad); k += blockDim.x) gbias[k] = G.bias[k];
         __syncthreads();
     }
-    const uint32_t count = Wv.counters[cur];
+    const uint32_t count = Wv.counters[qword(cur)];
     if (blockIdx.x == 0 && threadIdx.x < kParts) Wv.counters[part_word(threadIdx.x, P_FETCH_EXT)] = 0;  // for the next extend
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
@@ -963,7 +968,8 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                 n_extra = make_uint4(uint32_t(nst), uint32_t(nst >> 32), extra.z + 1u, extra.w);
             }
         }
-        uint32_t sh_pos = block_append(Wv.counters + C_SHADOW, emit_shadow ? 1u : 0u, scan);
+        uint32_t q_pos, sh_pos;
+        block_append2(Wv.counters + qword(cur ^ 1), cont ? 1u : 0u, emit_shadow ? 1u : 0u, scan, &q_pos, &sh_pos);
         if (emit_shadow) {
             Wv.sh_o[sh_pos] = sh_o;
             Wv.sh_d[sh_pos] = sh_d;
@@ -971,7 +977,6 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
             if (GUIDED || (!BATCH && R.layers > 0)) Wv.sh_tag[sh_pos] = sh_tag;
             if constexpr (BATCH) Wv.sh_job[sh_pos] = jid;
         }
-        uint32_t q_pos = block_append(Wv.counters + (cur ^ 1), cont ? 1u : 0u, scan);
         if (cont) {
             Wv.ray_o[cur ^ 1][q_pos] = n_o;
             Wv.ray_d[cur ^ 1][q_pos] = n_d;
@@ -987,10 +992,11 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
 // (7 wavefronts per SIMD: the any-hit leaf mode's state fits 72 VGPRs without spills; at 64 VGPRs
 // it spilled and ran 2 % slower)
 template <int TRAV, int STACK, bool BATCH>
-__global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv)
+__global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, const JobDev* __restrict__ jobs, WaveDev Wv,
+                                                                             int sh)
 {
     __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
-    const uint32_t count = Wv.counters[C_SHADOW];
+    const uint32_t count = Wv.counters[shword(sh)];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0)
         atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_SHADOW), (unsigned long long)count);
@@ -1104,11 +1110,11 @@ constexpr uint32_t kFilmLdsBytes = 48 * 1024;  // FP64 histograms: pix_slots * T
 constexpr uint32_t kFilmMaxSlots = 8;
 
 template <bool GUIDED>
-__global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev Wv, uint32_t pix_slots, float* contrib)
+__global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev Wv, int sh, uint32_t pix_slots, float* contrib)
 {
     extern __shared__ double hist[];         // pix_slots * T * 3 partials
     __shared__ uint32_t slot_pix[kFilmMaxSlots];
-    const uint32_t count = Wv.counters[C_SHADOW];
+    const uint32_t count = Wv.counters[shword(sh)];
     const uint32_t T = uint32_t(R.T), T3 = 3u * T, hsize = pix_slots * T3;
     // the histograms start at zero; each flush reads and clears only the slots that were used
     for (uint32_t i = threadIdx.x; i < hsize; i += blockDim.x) hist[i] = 0.0;
@@ -1164,9 +1170,9 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
 
 // multibounce films: visible connections mark their layer for the camera sample, add to the
 // steady image, and (inside the time window) to their layer's film
-__global__ void __launch_bounds__(kBlock) film_mb_kernel(RenderDev R, WaveDev Wv)
+__global__ void __launch_bounds__(kBlock) film_mb_kernel(RenderDev R, WaveDev Wv, int sh)
 {
-    const uint32_t count = Wv.counters[C_SHADOW];
+    const uint32_t count = Wv.counters[shword(sh)];
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count; r += gridDim.x * blockDim.x) {
         if (!Wv.sh_vis[r]) continue;
         const float4 c = Wv.sh_c[r];
@@ -1204,9 +1210,9 @@ __global__ void __launch_bounds__(kBlock) mb_weight_kernel(RenderDev R, WaveDev 
 }
 
 // batched renders: visible contributions go straight to their job's film, one compensated add each
-__global__ void __launch_bounds__(kBlock) film_batch_kernel(const JobDev* __restrict__ jobs, WaveDev Wv)
+__global__ void __launch_bounds__(kBlock) film_batch_kernel(const JobDev* __restrict__ jobs, WaveDev Wv, int sh)
 {
-    const uint32_t count = Wv.counters[C_SHADOW];
+    const uint32_t count = Wv.counters[shword(sh)];
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count; r += gridDim.x * blockDim.x) {
         if (!Wv.sh_vis[r]) continue;
         const RenderDev& R = jobs[Wv.sh_job[r]].R;
@@ -2332,7 +2338,7 @@ int tr_film_add_samples(tr_film f, const tr_film_sample* samples, uint32_t n)
         return rc;
     }
     uint32_t counts[C_WORDS] = {};
-    counts[C_SHADOW] = n;
+    counts[shword(0)] = n;
     RenderDev R{};
     R.film = f->d_sum;
     R.film_comp = f->d_comp;
@@ -2352,7 +2358,7 @@ int tr_film_add_samples(tr_film f, const tr_film_sample* samples, uint32_t n)
     if (e == hipSuccess) e = hipMemcpyAsync(dv, hv.data(), size_t(n), hipMemcpyHostToDevice, f->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dcount, counts, sizeof counts, hipMemcpyHostToDevice, f->stream);
     if (e == hipSuccess) {
-        film_kernel<false><<<fg, kFilmBlock, lds, f->stream>>>(R, Wv, pix_slots, nullptr);
+        film_kernel<false><<<fg, kFilmBlock, lds, f->stream>>>(R, Wv, 0, pix_slots, nullptr);
         e = hipGetLastError();
     }
     // the staging vectors above are read by the copies: wait before they go out of scope
@@ -2573,17 +2579,17 @@ struct Launch {
             }
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADOW))) return rc;
-            TR_TRACE_DISPATCH(trav_sh, stack_bound, batch, shadow_kernel, tg, kTraceBlock, stream, S0, dj, Wv);
+            TR_TRACE_DISPATCH(trav_sh, stack_bound, batch, shadow_kernel, tg, kTraceBlock, stream, S0, dj, Wv, cur ^ 1);
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_FILM))) return rc;
             if (batch)
-                film_batch_kernel<<<std::max(1u, std::min((wave + kBlock - 1) / kBlock, cu * 16u)), kBlock, 0, stream>>>(dj, Wv);
+                film_batch_kernel<<<std::max(1u, std::min((wave + kBlock - 1) / kBlock, cu * 16u)), kBlock, 0, stream>>>(dj, Wv, cur ^ 1);
             else if (R0.layers > 0)
-                film_mb_kernel<<<std::max(1u, std::min((wave + kBlock - 1) / kBlock, cu * 16u)), kBlock, 0, stream>>>(R0, Wv);
+                film_mb_kernel<<<std::max(1u, std::min((wave + kBlock - 1) / kBlock, cu * 16u)), kBlock, 0, stream>>>(R0, Wv, cur ^ 1);
             else if (guide)
-                film_kernel<true><<<fg, kFilmBlock, film_lds, stream>>>(R0, Wv, pix_slots, GV.contrib);
+                film_kernel<true><<<fg, kFilmBlock, film_lds, stream>>>(R0, Wv, cur ^ 1, pix_slots, GV.contrib);
             else
-                film_kernel<false><<<fg, kFilmBlock, film_lds, stream>>>(R0, Wv, pix_slots, nullptr);
+                film_kernel<false><<<fg, kFilmBlock, film_lds, stream>>>(R0, Wv, cur ^ 1, pix_slots, nullptr);
             if ((rc = end())) return rc;
             cur ^= 1;
         }
