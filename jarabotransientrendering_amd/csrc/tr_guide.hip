@@ -183,10 +183,17 @@ __device__ float loss_grad(const float* o, float wx, float wy, float wz, float w
     return -w * log_q;
 }
 
-// ---- training: one wavefront per block, 64 records per block ------------------------------------
+// ---- training: one wavefront per block, tiles of 64 records ------------------------------------
 // LDS: acts[5][64][64] bf16 neuron-major layer inputs (40 KiB), dY[64][64] bf16 neuron-major
-// (8 KiB), U: feature rows / raw outputs (10 KiB). Writes the block's gradient (sum over its
-// records) to partial[block][kParams] and its loss sum to partial_loss[block].
+// (8 KiB), U: feature rows / raw outputs (10 KiB). Block b takes tiles b, b + gridDim.x, ... of
+// the step's records and writes their gradient (sum over the records) to partial[b][kParams] and
+// their loss sum to partial_loss[b]; the block's first tile stores, later ones add (each lane reads
+// back only what it wrote). The weight-gradient tiles come out of the matrix cores row-major in the
+// parameter layout (W_l[i][o], lanes of a 16-lane group on consecutive o), so every store writes
+// 64-byte segments (with the transposed tile, every store instruction touched 64 lines: the step
+// ran 1.4 ms for 262,144 records). Launches use one block per tile (train_grid): a grid of two
+// blocks per CU that accumulated 8 tiles each ran 1.5x slower, its read-add-write of the partial
+// waiting on one dependent load per output tile.
 struct TrainArgs {
     const float4* rec;
     const uint32_t* count;   // records available (device)
@@ -200,16 +207,13 @@ struct TrainArgs {
     float* partial_loss;
 };
 
-__global__ void __launch_bounds__(64) train_kernel(TrainArgs A)
+// One 64-record tile of a training step: forward, loss gradient, backward; adds the tile's weight
+// and bias gradient into the block's partial P (stores when `add` is false) and returns its loss.
+__device__ __forceinline__ float train_tile(const TrainArgs& A, uint32_t r0, uint32_t total, uint32_t avail, bool add,
+                                            float* P, uint16_t (*acts)[64 * 64], uint16_t* dY, float* U)
 {
-    __shared__ uint16_t acts[kLayers][64 * 64];
-    __shared__ uint16_t dY[64 * 64];
-    __shared__ float U[64 * kOut];
     const int lane = int(__lane_id()), g = lane >> 4, c = lane & 15;
-    const uint32_t total = *A.count;
-    const uint32_t avail = total > A.first ? min(total - A.first, A.batch) : 0u;
-    const uint32_t r0 = blockIdx.x * 64u;
-    if (r0 >= avail) return;
+    float tile_loss = 0.f;
     const uint32_t ridx = r0 + uint32_t(lane);
     const bool valid = ridx < avail;
     float4 q0 = {}, q1 = {}, q2 = {}, q3 = {};
@@ -243,33 +247,35 @@ __global__ void __launch_bounds__(64) train_kernel(TrainArgs A)
         for (int k = 0; k < 64; ++k) dY[k * 64 + lane] = k < kOut ? uint16_t(bf16_bits(d[k])) : uint16_t(0);
         // wavefront loss sum
         for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-        if (lane == 0) A.partial_loss[blockIdx.x] = loss;
+        tile_loss = loss;
     }
     __syncthreads();
-    float* P = A.partial + size_t(blockIdx.x) * kParams;
     for (int l = kLayers - 1; l >= 0; --l) {
         const int n_in = kIn[l], n_out = kOutL[l];
         const uint16_t* X = acts[l];
-        // weight gradient D[o][i] = sum_s dY[o][s] X[i][s]; stored as P[W_l + i*n_out + o]
+        // weight gradient D[i][o] = sum_s X[i][s] dY[o][s], stored as P[W_l + i*n_out + o]: the
+        // output tile's rows are inputs i (lane group g, element j), its columns outputs o (lane c)
         for (int ot = 0; ot < (n_out + 15) / 16; ++ot)
             for (int it = 0; it < n_in / 16; ++it) {
                 f32x4 acc = {0.f, 0.f, 0.f, 0.f};
                 for (int ks = 0; ks < 2; ++ks) {
-                    uint4 a = *reinterpret_cast<const uint4*>(dY + (ot * 16 + c) * 64 + ks * 32 + 8 * g);
-                    uint4 b = *reinterpret_cast<const uint4*>(X + (it * 16 + c) * 64 + ks * 32 + 8 * g);
+                    uint4 a = *reinterpret_cast<const uint4*>(X + (it * 16 + c) * 64 + ks * 32 + 8 * g);
+                    uint4 b = *reinterpret_cast<const uint4*>(dY + (ot * 16 + c) * 64 + ks * 32 + 8 * g);
                     acc = mfma32(a, b, acc);
                 }
-                const int i = it * 16 + c;
-                for (int j = 0; j < 4; ++j) {
-                    int o = ot * 16 + 4 * g + j;
-                    if (o < n_out) P[param_offset(l) + i * n_out + o] = acc[j];
-                }
+                const int o = ot * 16 + c;
+                if (o < n_out)
+                    for (int j = 0; j < 4; ++j) {
+                        float* q = P + param_offset(l) + (it * 16 + 4 * g + j) * n_out + o;
+                        *q = add ? *q + acc[j] : acc[j];
+                    }
             }
         // bias gradient
         if (lane < n_out) {
             float sb = 0.f;
             for (int s = 0; s < 64; ++s) sb += bf16_to_f32(dY[lane * 64 + s]);
-            P[param_offset(l) + n_in * n_out + lane] = sb;
+            float* q = P + param_offset(l) + n_in * n_out + lane;
+            *q = add ? *q + sb : sb;
         }
         if (l == 0) break;
         // input gradient dX[i][s] = sum_o W[i][o] dY[o][s], masked by ReLU'(layer input)
@@ -302,20 +308,43 @@ __global__ void __launch_bounds__(64) train_kernel(TrainArgs A)
                 }
         __syncthreads();
     }
+    return tile_loss;
 }
 
-// First stage of the gradient sum: block row y adds the per-chunk gradients of its slice of the
-// step's chunks, in chunk order, so the sum is deterministic and the Adam kernel adds only
-// kReduceSlices values per parameter (it summed every chunk serially, 256 for 16,384 records).
+__global__ void __launch_bounds__(64) train_kernel(TrainArgs A)
+{
+    __shared__ uint16_t acts[kLayers][64 * 64];
+    __shared__ uint16_t dY[64 * 64];
+    __shared__ float U[64 * kOut];
+    const uint32_t total = *A.count;
+    const uint32_t avail = total > A.first ? min(total - A.first, A.batch) : 0u;
+    float* P = A.partial + size_t(blockIdx.x) * kParams;
+    float loss = 0.f;
+    for (uint32_t r0 = blockIdx.x * 64u; r0 < avail; r0 += gridDim.x * 64u) {
+        loss += train_tile(A, r0, total, avail, r0 != blockIdx.x * 64u, P, acts, dY, U);
+        __syncthreads();  // the next tile overwrites the LDS arrays
+    }
+    if (__lane_id() == 0 && blockIdx.x * 64u < avail) A.partial_loss[blockIdx.x] = loss;
+}
+
+// Blocks of a training step: one per tile of 64 records
+inline uint32_t train_grid(uint32_t records)
+{
+    return std::max(1u, (records + 63u) / 64u);
+}
+
+// First stage of the gradient sum: block row y adds the partials of its slice of the training
+// grid's blocks (those that had a tile), in block order, so the sum is deterministic and the Adam
+// kernel adds only kReduceSlices values per parameter.
 constexpr uint32_t kReduceSlices = 32;
 
 __global__ void reduce_kernel(const float* __restrict__ partial, const float* __restrict__ partial_loss,
-                              const uint32_t* count, uint32_t first, uint32_t batch, float* __restrict__ part2,
-                              float* __restrict__ loss2)
+                              const uint32_t* count, uint32_t first, uint32_t batch, uint32_t train_blocks,
+                              float* __restrict__ part2, float* __restrict__ loss2)
 {
     const uint32_t total = *count;
     const uint32_t avail = total > first ? min(total - first, batch) : 0u;
-    const uint32_t chunks = (avail + 63u) / 64u;
+    const uint32_t chunks = min((avail + 63u) / 64u, train_blocks);
     const uint32_t per = (chunks + kReduceSlices - 1u) / kReduceSlices;
     const uint32_t c0 = min(blockIdx.y * per, chunks), c1 = min(c0 + per, chunks);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -519,12 +548,13 @@ struct tr_guide_s {
             T.bias = bias;
             T.partial = partial;
             T.partial_loss = partial_loss;
-            train_kernel<<<max_chunks, 64, 0, s>>>(T);
+            const uint32_t tg = train_grid(desc.batch_size);
+            train_kernel<<<tg, 64, 0, s>>>(T);
             GHIP_TRY(hipGetLastError());
             ++steps;
             AdamArgs A;
             reduce_kernel<<<dim3((kParams + 255) / 256, kReduceSlices), 256, 0, s>>>(partial, partial_loss, count, T.first,
-                                                                                     T.batch, part2, loss2);
+                                                                                     T.batch, tg, part2, loss2);
             GHIP_TRY(hipGetLastError());
             A.master = master, A.m = m, A.v = v, A.partial = part2, A.partial_loss = loss2;
             A.count = count, A.first = T.first, A.batch = T.batch;
@@ -624,7 +654,7 @@ int tr_guide_create(tr_guide* out, const tr_guide_desc* d)
     tr_guide g = new (std::nothrow) tr_guide_s;
     if (!g) return tri::fail(TR_ERR_NOMEM, "tr_guide_create: out of host memory");
     g->desc = D;
-    g->max_chunks = D.batch_size / 64;
+    g->max_chunks = train_grid(D.batch_size);  // partials: one per training block
     auto bail = [&](int code) {
         delete g;
         return code;
@@ -760,9 +790,10 @@ int tr_guide_gradient(tr_guide g, const float* records, uint32_t n, float* grad,
     T.frag = reinterpret_cast<const uint4*>(g->fwd);
     T.bfrag = reinterpret_cast<const uint4*>(g->bwd);
     T.bias = g->bias, T.partial = g->partial, T.partial_loss = g->partial_loss;
-    train_kernel<<<(n + 63) / 64, 64, 0, g->stream>>>(T);
+    const uint32_t tg = train_grid(n);
+    train_kernel<<<tg, 64, 0, g->stream>>>(T);
     hipError_t e = hipGetLastError();
-    std::vector<float> part(size_t((n + 63) / 64) * kParams), pl((n + 63) / 64);
+    std::vector<float> part(size_t(tg) * kParams), pl(tg);
     if (e == hipSuccess)
         e = hipMemcpyAsync(part.data(), g->partial, part.size() * sizeof(float), hipMemcpyDeviceToHost, g->stream);
     if (e == hipSuccess)

@@ -17,6 +17,7 @@
 // bounce k are records [0, count) of buffer k & 1, and shade writes the paths that continue,
 // compacted in queue order, into the other buffer, so every kernel reads and writes its records
 // densely. Queue counters live in device memory so no host round trip is needed between kernels.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,7 +46,7 @@ constexpr int kBlock = 256;        // other kernels
 constexpr int kShadeBlock = 1024;  // shade: one queue atomic per 1024 paths
 constexpr uint32_t kDefaultWave = 1u << 25;
 constexpr uint32_t kFetchMax = 512;  // rays per work fetch of a traversal wavefront (upper bound)
-constexpr uint32_t kTraceRaysPerThread = 16;  // lower bound on a full wavefront's rays per traversal thread
+constexpr uint32_t kTraceRaysPerThread = 8;   // lower bound on a full wavefront's rays per traversal thread
 constexpr uint32_t kTraceMinPerCu = 4;        // traversal blocks per CU below which that bound yields
 
 // Rays per work fetch: large enough that same-address fetch atomics stay rare, small enough that
@@ -667,14 +668,15 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
     __shared__ float glds[GUIDED ? (kShadeBlockGuided / 64) * 64 * trg::kRow : 1];
     __shared__ uint4 gfrag[GUIDED ? trg::kFwdFrags * 64 : 1];
     __shared__ float gbias[GUIDED ? trg::kBiasPad : 1];
+    const uint32_t count = Wv.counters[qword(cur)];
+    if (blockIdx.x == 0 && threadIdx.x < kParts) Wv.counters[part_word(threadIdx.x, P_FETCH_EXT)] = 0;  // for the next extend
+    if (blockIdx.x * blockDim.x >= count) return;  // no paths for this block (the last bounces' launches)
     if constexpr (GUIDED) {
         // the network's weights stay in LDS for the whole block (38 KiB + biases)
         for (uint32_t k = threadIdx.x; k < uint32_t(trg::kFwdFrags * 64); k += blockDim.x) gfrag[k] = G.frag[k];
         for (uint32_t k = threadIdx.x; k < uint32_t(trg::kBiasPad); k += blockDim.x) gbias[k] = G.bias[k];
         __syncthreads();
     }
-    const uint32_t count = Wv.counters[qword(cur)];
-    if (blockIdx.x == 0 && threadIdx.x < kParts) Wv.counters[part_word(threadIdx.x, P_FETCH_EXT)] = 0;  // for the next extend
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
@@ -1115,6 +1117,7 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
     extern __shared__ double hist[];         // pix_slots * T * 3 partials
     __shared__ uint32_t slot_pix[kFilmMaxSlots];
     const uint32_t count = Wv.counters[shword(sh)];
+    if (blockIdx.x * kFilmRange >= count) return;  // no range for this block (the last bounces' launches)
     const uint32_t T = uint32_t(R.T), T3 = 3u * T, hsize = pix_slots * T3;
     // the histograms start at zero; each flush reads and clears only the slots that were used
     for (uint32_t i = threadIdx.x; i < hsize; i += blockDim.x) hist[i] = 0.0;
@@ -1312,12 +1315,23 @@ __global__ void __launch_bounds__(kTraceBlock) occluded_kernel(SceneDev S, const
 // The traversal stack holds at most one entry per BVH level; the smallest instantiation that
 // covers the scene's depth keeps LDS per block (and so occupancy) in check.
 // `bound` = entries the traversal stack can hold at most (binary: depth; 4-wide: 3 per level).
-#define TR_STACK_DISPATCH3(T, bound, B, KERNEL, GRID, BLOCK, STREAM, ...)                \
-    do {                                                                                   \
-        if ((bound) < 16) KERNEL<T, 16, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);       \
-        else if ((bound) < 24) KERNEL<T, 24, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);  \
-        else if ((bound) < 32) KERNEL<T, 32, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);  \
-        else KERNEL<T, kBvhStack, B><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);             \
+// Launches kernel k on `s`; with a start event, through hipExtLaunchKernelGGL, which stamps the
+// start and stop events as part of the dispatch itself instead of as marker packets around it (a
+// marker pair between two launches drains the GPU: 3 % of a media-workload render timed this way).
+template <class... P, class... A>
+void launch_timed(void (*k)(P...), uint32_t grid, uint32_t block, hipStream_t s, hipEvent_t e0, hipEvent_t e1, A... a)
+{
+    if (e0) hipExtLaunchKernelGGL(k, dim3(grid), dim3(block), 0u, s, e0, e1, 0u, P(a)...);
+    else k<<<grid, block, 0, s>>>(a...);
+}
+// (the extend / shadow dispatch macros below run inside LaunchSet, whose ev_start / ev_stop are
+// the current launch's events, or null)
+#define TR_STACK_DISPATCH3(T, bound, B, KERNEL, GRID, BLOCK, STREAM, ...)                                   \
+    do {                                                                                                    \
+        if ((bound) < 16) launch_timed(KERNEL<T, 16, B>, GRID, BLOCK, STREAM, ev_start, ev_stop, __VA_ARGS__); \
+        else if ((bound) < 24) launch_timed(KERNEL<T, 24, B>, GRID, BLOCK, STREAM, ev_start, ev_stop, __VA_ARGS__); \
+        else if ((bound) < 32) launch_timed(KERNEL<T, 32, B>, GRID, BLOCK, STREAM, ev_start, ev_stop, __VA_ARGS__); \
+        else launch_timed(KERNEL<T, kBvhStack, B>, GRID, BLOCK, STREAM, ev_start, ev_stop, __VA_ARGS__);      \
     } while (0)
 #define TR_STACK_DISPATCH3B(T, bound, batch, KERNEL, GRID, BLOCK, STREAM, ...)                             \
     do {                                                                                                   \
@@ -2492,6 +2506,8 @@ struct Launch {
     bool events_on = false;
     bool extend_only = false;  // events around the extend launches only
     bool open = false;         // begin() recorded an event that end() closes
+    // events of the current extend / shadow launch, stamped by the launch itself (launch_timed)
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 
     int flush_events()
     {
@@ -2516,7 +2532,12 @@ struct Launch {
         }
         int slot = int(ev_kind.size());
         ev_kind.push_back({kind, slot});
-        HIP_TRY(hipEventRecord(ev[2 * slot], stream));
+        if (kind == TR_K_EXTEND || kind == TR_K_SHADOW) {
+            ev_start = ev[2 * slot];
+            ev_stop = ev[2 * slot + 1];
+        } else {
+            HIP_TRY(hipEventRecord(ev[2 * slot], stream));
+        }
         return TR_OK;
     }
     int end()
@@ -2524,6 +2545,10 @@ struct Launch {
         HIP_TRY(hipGetLastError());
         if (!open) return TR_OK;
         open = false;
+        if (ev_start) {
+            ev_start = ev_stop = nullptr;
+            return TR_OK;
+        }
         HIP_TRY(hipEventRecord(ev[2 * ev_kind.back().second + 1], stream));
         return TR_OK;
     }
@@ -2543,9 +2568,9 @@ struct Launch {
         const uint32_t wave = c.wave, cu = uint32_t(std::max(compute_units, 1));
         // persistent traversal grid: trace_per_cu blocks per CU, but at least kTraceRaysPerThread rays
         // per thread of a full wavefront (a smaller scene's wavefronts run faster on fewer, longer-lived
-        // blocks: 1.6x on the 2-M-path media workload, no change on 33-M-path wavefronts) as long as
-        // every CU keeps kTraceMinPerCu blocks (small renders: 128 blocks for a 262,144-path wavefront
-        // left three quarters of the SIMDs idle)
+        // blocks: 8 rays per thread is 8 % faster than 4 or 16 on the 2-M-path media workload, no
+        // change on 33-M-path wavefronts) as long as every CU keeps kTraceMinPerCu blocks (small
+        // renders: 128 blocks for a 262,144-path wavefront left three quarters of the SIMDs idle)
         const uint32_t by_rays = std::max((wave + kTraceBlock * kTraceRaysPerThread - 1) / (kTraceBlock * kTraceRaysPerThread),
                                           cu * kTraceMinPerCu);
         const uint32_t tg = std::max(1u, std::min({(wave + kTraceBlock - 1) / kTraceBlock, cu * c.trace_per_cu, by_rays}));
