@@ -108,7 +108,7 @@ struct SceneDev {
 struct RenderDev {
     // film
     float* film;               // compensated sums [rows][W][T][3]: sum ...
-    float* film_comp;          // ... and the rounding errors of its additions (film_add)
+    float* film_comp;          // ... and the rounding errors of its additions (film_add3)
     int32_t W, H, T, row0, spp, spp2, n_inc;
     float exposure, offset, total_time, inv_inc;
     int32_t unwarp, max_bounces;
@@ -1077,12 +1077,19 @@ __device__ __forceinline__ float two_sum_err(float a, float b, float s)
     return (a - (s - bb)) + (b - bb);
 }
 
-// x into the compensated pair (sum[i], comp[i]) held in global memory
-__device__ __forceinline__ void film_add(float* __restrict__ sum, float* __restrict__ comp, size_t i, float x)
+// an RGB contribution into elements i, i + 1, i + 2 of the compensated pair (sum, comp) held in
+// global memory: the three returning atomics are issued before the first result is awaited (one
+// round trip instead of three)
+__device__ __forceinline__ void film_add3(float* __restrict__ sum, float* __restrict__ comp, size_t i, float x,
+                                          float y, float z)
 {
-    const float a = unsafeAtomicAdd(sum + i, x);
-    const float e = two_sum_err(a, x, a + x);
-    if (e != 0.0f) unsafeAtomicAdd(comp + i, e);
+    const float a0 = unsafeAtomicAdd(sum + i, x);
+    const float a1 = unsafeAtomicAdd(sum + i + 1, y);
+    const float a2 = unsafeAtomicAdd(sum + i + 2, z);
+    const float e0 = two_sum_err(a0, x, a0 + x), e1 = two_sum_err(a1, y, a1 + y), e2 = two_sum_err(a2, z, a2 + z);
+    if (e0 != 0.0f) unsafeAtomicAdd(comp + i, e0);
+    if (e1 != 0.0f) unsafeAtomicAdd(comp + i + 1, e1);
+    if (e2 != 0.0f) unsafeAtomicAdd(comp + i + 2, e2);
 }
 
 // a compensated partial (s, c) into the pair (sum[i], comp[i])
@@ -1150,10 +1157,7 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
                 atomicAdd(h + 1, double(c.y));
                 atomicAdd(h + 2, double(c.z));
             } else {
-                const size_t e = (size_t(pix) * T + bin) * 3;
-                film_add(R.film, R.film_comp, e + 0, c.x);
-                film_add(R.film, R.film_comp, e + 1, c.y);
-                film_add(R.film, R.film_comp, e + 2, c.z);
+                film_add3(R.film, R.film_comp, (size_t(pix) * T + bin) * 3, c.x, c.y, c.z);
             }
         }
         __syncthreads();
@@ -1183,13 +1187,9 @@ __global__ void __launch_bounds__(kBlock) film_mb_kernel(RenderDev R, WaveDev Wv
         const uint32_t tag = Wv.sh_tag[r], pix = __float_as_uint(c.w), fidx = __float_as_uint(Wv.sh_d[r].w);
         atomicOr(R.cs_bits + (tag >> kMaxLayersTagBits), 1u << (tag & ((1u << kMaxLayersTagBits) - 1u)));
         if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
-            film_add(R.steady, R.steady_comp, size_t(pix) * 3 + 0, c.x);
-            film_add(R.steady, R.steady_comp, size_t(pix) * 3 + 1, c.y);
-            film_add(R.steady, R.steady_comp, size_t(pix) * 3 + 2, c.z);
+            film_add3(R.steady, R.steady_comp, size_t(pix) * 3, c.x, c.y, c.z);
             if (fidx != 0xFFFFFFFFu) {
-                film_add(R.film, R.film_comp, size_t(fidx) * 3 + 0, c.x);
-                film_add(R.film, R.film_comp, size_t(fidx) * 3 + 1, c.y);
-                film_add(R.film, R.film_comp, size_t(fidx) * 3 + 2, c.z);
+                film_add3(R.film, R.film_comp, size_t(fidx) * 3, c.x, c.y, c.z);
             }
         }
     }
@@ -1223,10 +1223,7 @@ __global__ void __launch_bounds__(kBlock) film_batch_kernel(const JobDev* __rest
         float4 c = Wv.sh_c[r];
         uint32_t pix = __float_as_uint(c.w);
         uint32_t bin = __float_as_uint(Wv.sh_d[r].w) - pix * uint32_t(R.T);
-        const size_t e = (size_t(pix) * uint32_t(R.T) + bin) * 3;
-        film_add(R.film, R.film_comp, e + 0, c.x);
-        film_add(R.film, R.film_comp, e + 1, c.y);
-        film_add(R.film, R.film_comp, e + 2, c.z);
+        film_add3(R.film, R.film_comp, (size_t(pix) * uint32_t(R.T) + bin) * 3, c.x, c.y, c.z);
     }
 }
 
