@@ -97,6 +97,25 @@ __device__ __forceinline__ f32x4 mfma32(uint4 a, uint4 b, f32x4 c)
 //   t/total_time and sin/cos(2^f * pi * t) for f = 0..4                                11
 //   medium flag                                                                         1
 //   zero padding                                                                        3
+// sin(pi 2^q x), cos(pi 2^q x) for q = 0 .. Q-1 into f[2q], f[2q + 1]: one sincospi, then the
+// double-angle identities sin 2a = 2 sin a cos a, cos 2a = 1 - 2 sin^2 a (each doubling at most
+// doubles the absolute error, so the top octave is within ~2^Q ulp, far below the bf16 rounding
+// the network's inputs get)
+template <int Q>
+__device__ __forceinline__ void octaves(float x, float* f)
+{
+    float sn, cs;
+    sincospif(x, &sn, &cs);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        f[2 * q] = sn;
+        f[2 * q + 1] = cs;
+        const float s2 = 2.0f * sn * cs, c2 = fmaf(-2.0f * sn, sn, 1.0f);
+        sn = s2;
+        cs = c2;
+    }
+}
+
 __device__ __forceinline__ void encode_features(const float* raw, float* f)
 {
     int k = 0;
@@ -104,30 +123,14 @@ __device__ __forceinline__ void encode_features(const float* raw, float* f)
     for (int c = 0; c < 3; ++c) f[k++] = raw[c];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        float s = 1.0f;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float sn, cs;
-            sincospif(s * raw[c], &sn, &cs);
-            f[k++] = sn;
-            f[k++] = cs;
-            s *= 2.0f;
-        }
+        octaves<4>(raw[c], f + k);
+        k += 8;
     }
 #pragma unroll
     for (int c = 3; c < 9; ++c) f[k++] = raw[c];
     f[k++] = raw[9];
-    {
-        float s = 1.0f;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-            float sn, cs;
-            sincospif(s * raw[9], &sn, &cs);
-            f[k++] = sn;
-            f[k++] = cs;
-            s *= 2.0f;
-        }
-    }
+    octaves<5>(raw[9], f + k);
+    k += 10;
     f[k++] = raw[10];
     while (k < kFeat) f[k++] = 0.0f;
 }

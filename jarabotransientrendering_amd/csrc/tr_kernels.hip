@@ -2574,7 +2574,10 @@ struct Launch {
         const uint32_t tg = std::max(1u, std::min({(wave + kTraceBlock - 1) / kTraceBlock, cu * c.trace_per_cu, by_rays}));
         const uint32_t fg = std::max(1u, std::min((wave + kFilmRange - 1) / kFilmRange, cu * 4u));
         const uint32_t sblock = c.mode == kShadeGuided ? kShadeBlockGuided : kShadeBlock;
-        const uint32_t scap = cu * (c.mode == kShadeGuided ? 8u : c.shade_per_cu);
+        // guided shading: 2 blocks per CU (its LDS allows no more), persistent over the wavefront, so
+        // each block stages the network's 38 KB of weights once per launch instead of once per
+        // 256 paths
+        const uint32_t scap = cu * (c.mode == kShadeGuided ? 2u : c.shade_per_cu);
         const uint32_t sg = std::max(1u, std::min((wave + sblock - 1) / sblock, scap));
         const uint32_t cg = (wave + kBlock - 1) / kBlock;
         if ((rc = begin(TR_K_CAMERA))) return rc;
