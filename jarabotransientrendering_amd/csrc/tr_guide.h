@@ -244,62 +244,76 @@ __device__ __forceinline__ float softplus(float x) { return x > 15.f ? x : log1p
 constexpr float kKappaMin = 1e-2f;
 constexpr float kKappaMax = 2e3f;
 
-// vMF density kappa / (2 pi (1 - exp(-2 kappa))) * exp(kappa (mu.w - 1)), stable for small kappa
-__device__ __forceinline__ float vmf_pdf(const Lobe& l, float wx, float wy, float wz)
-{
-    float c = l.mx * wx + l.my * wy + l.mz * wz;
-    float norm = l.kappa / (2.0f * 3.14159265358979f * -expm1f(-2.0f * l.kappa));
-    return norm * __expf(l.kappa * (c - 1.0f));
-}
-
-// The same mixture read lobe by lobe from the 40 raw outputs (kept in LDS), so that the caller
-// holds no per-lobe registers: softmax normaliser first, then lobe i on demand.
-struct MixtureRef {
-    const float* o;
-    float logit_max, inv_sum;
+// The mixture of one sample, decoded once from its 40 raw outputs (kept in LDS): per lobe the unit
+// mean direction, kappa, the vMF density's normaliser kappa / (2 pi (1 - e^{-2 kappa})) (through
+// expm1, stable for small kappa; the density is norm * exp(kappa (mu.w - 1))) and the softmax
+// weight, in registers, so that picking a lobe, sampling it and evaluating the mixture density
+// decode nothing again.
+struct Mixture {
+    float mx[kLobes], my[kLobes], mz[kLobes], kappa[kLobes], norm[kLobes], w[kLobes];
 };
-__device__ __forceinline__ MixtureRef mixture_ref(const float* o)
+__device__ __forceinline__ void mixture_decode(const float* o, Mixture& M)
 {
-    MixtureRef M{o, -3.4e38f, 0.f};
-    for (int i = 0; i < kLobes; ++i) M.logit_max = fmaxf(M.logit_max, o[5 * i + 4]);
-    float s = 0.f;
-    for (int i = 0; i < kLobes; ++i) s += __expf(o[5 * i + 4] - M.logit_max);
-    M.inv_sum = 1.0f / s;
-    return M;
-}
-__device__ __forceinline__ Lobe lobe_at(const MixtureRef& M, int i)
-{
-    Lobe L;
-    float x = M.o[5 * i], y = M.o[5 * i + 1], z = M.o[5 * i + 2];
-    float r = sqrtf(x * x + y * y + z * z);
-    if (r > 1e-20f) {
-        float inv = 1.0f / r;
-        L.mx = x * inv, L.my = y * inv, L.mz = z * inv;
-    } else {
-        L.mx = 0.f, L.my = 0.f, L.mz = 1.f;
+    float lmax = -3.4e38f;
+#pragma unroll
+    for (int i = 0; i < kLobes; ++i) lmax = fmaxf(lmax, o[5 * i + 4]);
+    float sum = 0.f;
+#pragma unroll
+    for (int i = 0; i < kLobes; ++i) {
+        M.w[i] = __expf(o[5 * i + 4] - lmax);
+        sum += M.w[i];
     }
-    L.kappa = fminf(softplus(M.o[5 * i + 3]) + kKappaMin, kKappaMax);
-    L.w = __expf(M.o[5 * i + 4] - M.logit_max) * M.inv_sum;
+    const float inv_sum = 1.0f / sum;
+#pragma unroll
+    for (int i = 0; i < kLobes; ++i) {
+        M.w[i] *= inv_sum;
+        const float x = o[5 * i], y = o[5 * i + 1], z = o[5 * i + 2];
+        const float r2 = x * x + y * y + z * z;
+        if (r2 > 1e-38f) {
+            const float ir = rsqrtf(r2);
+            M.mx[i] = x * ir, M.my[i] = y * ir, M.mz[i] = z * ir;
+        } else {
+            M.mx[i] = 0.f, M.my[i] = 0.f, M.mz[i] = 1.f;
+        }
+        const float k = fminf(softplus(o[5 * i + 3]) + kKappaMin, kKappaMax);
+        M.kappa[i] = k;
+        M.norm[i] = k / (2.0f * 3.14159265358979f * -expm1f(-2.0f * k));
+    }
+}
+// lobe i of the mixture (i uniform or not: selected without indexing the register arrays)
+__device__ __forceinline__ Lobe lobe_of(const Mixture& M, int i)
+{
+    Lobe L{M.mx[0], M.my[0], M.mz[0], M.kappa[0], M.w[0]};
+#pragma unroll
+    for (int j = 1; j < kLobes; ++j)
+        if (i == j) L = Lobe{M.mx[j], M.my[j], M.mz[j], M.kappa[j], M.w[j]};
     return L;
 }
-__device__ __forceinline__ float mixture_pdf(const MixtureRef& M, float wx, float wy, float wz)
+__device__ __forceinline__ float mixture_pdf(const Mixture& M, float wx, float wy, float wz)
 {
     float p = 0.f;
+#pragma unroll
     for (int i = 0; i < kLobes; ++i) {
-        Lobe L = lobe_at(M, i);
-        p += L.w * vmf_pdf(L, wx, wy, wz);
+        const float c = M.mx[i] * wx + M.my[i] * wy + M.mz[i] * wz;
+        p += M.w[i] * M.norm[i] * __expf(M.kappa[i] * (c - 1.0f));
     }
     return p;
 }
 // lobe index for a uniform number u (inverse CDF of the weights)
-__device__ __forceinline__ int pick_lobe(const MixtureRef& M, float u)
+__device__ __forceinline__ int pick_lobe(const Mixture& M, float u)
 {
     float acc = 0.f;
+    int k = kLobes - 1;
+    bool found = false;
+#pragma unroll
     for (int i = 0; i < kLobes - 1; ++i) {
-        acc += __expf(M.o[5 * i + 4] - M.logit_max) * M.inv_sum;
-        if (u < acc) return i;
+        acc += M.w[i];
+        if (!found && u < acc) {
+            k = i;
+            found = true;
+        }
     }
-    return kLobes - 1;
+    return k;
 }
 
 // Sample a vMF lobe (W. Jakob, "Numerically stable sampling of the von Mises-Fisher distribution

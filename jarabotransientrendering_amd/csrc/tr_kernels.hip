@@ -929,11 +929,12 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                     float ct;
                     nd = medium_event ? sample_isotropic(d, e1, e2) : sample_lambert_onb(nside, e1, e2, &ct);
                 }
-                const trg::MixtureRef mix = trg::mixture_ref(mine + __lane_id() * trg::kRow);
+                trg::Mixture mix;
+                trg::mixture_decode(mine + __lane_id() * trg::kRow, mix);
                 if (!(usel < alpha)) {
                     float ul = pcg_float(pst, pinc);
                     e1 = pcg_float(pst, pinc), e2 = pcg_float(pst, pinc);
-                    trg::Lobe L = trg::lobe_at(mix, trg::pick_lobe(mix, ul));
+                    trg::Lobe L = trg::lobe_of(mix, trg::pick_lobe(mix, ul));
                     trg::vmf_sample(L, e1, e2, &nd.x, &nd.y, &nd.z);
                 }
                 float q = trg::mixture_pdf(mix, nd.x, nd.y, nd.z);
