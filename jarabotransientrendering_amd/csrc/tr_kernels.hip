@@ -293,8 +293,9 @@ constexpr int stack_kind() { return TRAV == kTravWide ? kStack32 : TRAV == kTrav
 
 // Per-lane ray refill for the persistent 4-wide traversal kernels: a lane whose query is complete
 // takes the next ray at once instead of waiting for the wavefront's slowest ray. The wavefront
-// claims `chunk` queue entries at a time from the fetch counter and hands them to its idle lanes
-// in lane order. Call with every lane of the wavefront; `busy` marks the lanes with a query.
+// claims `chunk` queue entries at a time (its static share first, then through its partition's
+// fetch cursor) and hands them to its idle lanes in lane order. Call with every lane of the
+// wavefront; `busy` marks the lanes with a query.
 constexpr int kRefillMin = 24;  // idle lanes that trigger a refill (unless the whole wavefront is idle)
 // Share of its even split of the queue that each wavefront takes without an atomic, in quarters
 // (0: one fetch chunk). Larger static shares measured slower on the 82k-triangle scene, whose rays'
