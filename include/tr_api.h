@@ -318,7 +318,7 @@ typedef struct {
     uint32_t wave_paths;    /* paths per wavefront (0 = library default) */
     uint32_t flags;         /* TR_RENDER_PROFILE | TR_RENDER_BINARY_BVH | TR_RENDER_ORTHONORMAL_FRAME | TR_RENDER_GRAPH
                              * | TR_RENDER_STACK32 | TR_RENDER_STACK_KEYED | TR_RENDER_PER_JOB */
-    uint32_t trace_per_cu;  /* persistent traversal blocks per compute unit (0 = default 16) */
+    uint32_t trace_per_cu;  /* persistent traversal blocks per compute unit (0 = default 24) */
     uint32_t shade_per_cu;  /* shading blocks per compute unit (0 = default 2) */
 } tr_launch_desc;
 
