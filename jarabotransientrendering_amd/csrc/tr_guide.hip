@@ -184,16 +184,17 @@ __device__ float loss_grad(const float* o, float wx, float wy, float wz, float w
 }
 
 // ---- training: one wavefront per block, tiles of 64 records ------------------------------------
-// LDS: acts[5][64][64] bf16 neuron-major layer inputs (40 KiB), dY[64][64] bf16 neuron-major
-// (8 KiB), U: feature rows / raw outputs (10 KiB). Block b takes tiles b, b + gridDim.x, ... of
-// the step's records and writes their gradient (sum over the records) to partial[b][kParams] and
-// their loss sum to partial_loss[b]; the block's first tile stores, later ones add (each lane reads
-// back only what it wrote). The weight-gradient tiles come out of the matrix cores row-major in the
-// parameter layout (W_l[i][o], lanes of a 16-lane group on consecutive o), so every store writes
-// 64-byte segments (with the transposed tile, every store instruction touched 64 lines: the step
-// ran 1.4 ms for 262,144 records). Launches use one block per tile (train_grid): a grid of two
-// blocks per CU that accumulated 8 tiles each ran 1.5x slower, its read-add-write of the partial
-// waiting on one dependent load per output tile.
+// LDS: acts[5][64][64] bf16 neuron-major layer inputs (40 KiB), U: feature rows / raw outputs
+// (10 KiB), and dY[64][64] bf16 neuron-major (8 KiB) in U's memory once the outputs are read:
+// 50 KiB per block, so three blocks fit on a CU (with 58 KiB, two did: the kernel waits on LDS and
+// memory more than it computes, and the third resident wavefront made it 21 % faster).
+// Block b takes tiles b, b + gridDim.x, ... of the step's records and writes their gradient (sum
+// over the records) to partial[b][kParams] and their loss sum to partial_loss[b]; the block's first
+// tile stores, later ones add (each lane reads back only what it wrote). Launches use one block per
+// tile (train_grid): a grid of two blocks per CU that accumulated 8 tiles each ran 1.5x slower, its
+// read-add-write of the partial waiting on one dependent load per output tile. The weight-gradient
+// tiles come out of the matrix cores row-major in the parameter layout (W_l[i][o], lanes of a
+// 16-lane group on consecutive o), so every store writes 64-byte segments.
 struct TrainArgs {
     const float4* rec;
     const uint32_t* count;   // records available (device)
@@ -238,6 +239,7 @@ __device__ __forceinline__ float train_tile(const TrainArgs& A, uint32_t r0, uin
     {
         float o[kOut], d[kOut];
         for (int k = 0; k < kOut; ++k) o[k] = U[lane * kOut + k];
+        __syncthreads();  // every output read before dY (the same LDS) is written
         float loss = 0.f;
         if (w > 0.f) {
             loss = loss_grad(o, q3.x, q3.y, q3.z, w, d);
@@ -314,8 +316,12 @@ __device__ __forceinline__ float train_tile(const TrainArgs& A, uint32_t r0, uin
 __global__ void __launch_bounds__(64) train_kernel(TrainArgs A)
 {
     __shared__ uint16_t acts[kLayers][64 * 64];
-    __shared__ uint16_t dY[64 * 64];
+    // U (feature rows, then the raw outputs) and dY share their LDS: the outputs are read into
+    // registers before the first output gradient is written, so a block needs 50 KB instead of
+    // 58 KB (three blocks per CU instead of two)
     __shared__ float U[64 * kOut];
+    static_assert(64 * 64 * sizeof(uint16_t) <= sizeof(U), "dY fits in U");
+    uint16_t* dY = reinterpret_cast<uint16_t*>(U);
     const uint32_t total = *A.count;
     const uint32_t avail = total > A.first ? min(total - A.first, A.batch) : 0u;
     float* P = A.partial + size_t(blockIdx.x) * kParams;
