@@ -1112,15 +1112,16 @@ __device__ __forceinline__ void film_add_partial(float* __restrict__ sum, float*
 // compensated add; records of further pixels go straight to HBM. The LDS histograms hold FP64
 // partials (non-returning LDS atomic adds): a partial of at most kFilmRange float contributions is
 // exact to within 2^-44 of their magnitude sum, far inside the film's compensated FP32 bound, and
-// is flushed as the float pair (hi, lo) = (float(p), float(p - hi)). Short ranges (512 records,
-// two 1024-bin pixels) measured best: 1.6x faster than 4096-record ranges on the 1024-bin films,
+// is flushed as the float pair (hi, lo) = (float(p), float(p - hi)). Short ranges (512 records)
+// measured best: 1.6x faster than 4096-record ranges on the 1024-bin films,
 // whose per-pixel records spread over many bins. The kernel waits on its loads and returning
-// atomics: 1024-thread blocks (two per CU with 48 KB of LDS each, 32 wavefronts) ran it 12-16 %
-// faster than 256-thread blocks (three per CU, 12 wavefronts).
+// atomics: 1024-thread blocks (two per CU, 32 wavefronts) ran it 12-16 % faster than 256-thread
+// blocks (three per CU, 12 wavefronts), and 72 KB of LDS per block (three 1024-bin pixels instead
+// of two, still two blocks per CU) 10-14 % faster again: fewer records of a range miss a slot.
 // (StreakCameraFilm::add_samples / add_temporal_sample, StreakCameraFilm.h:139-192, 280-297)
 constexpr int kFilmBlock = 1024;
 constexpr uint32_t kFilmRange = 512;
-constexpr uint32_t kFilmLdsBytes = 48 * 1024;  // FP64 histograms: pix_slots * T * 3 * 8 bytes
+constexpr uint32_t kFilmLdsBytes = 72 * 1024;  // FP64 histograms: pix_slots * T * 3 * 8 bytes (3 pixels of 1024 bins)
 constexpr uint32_t kFilmMaxSlots = 8;
 
 template <bool GUIDED>
