@@ -559,3 +559,13 @@ std::vector<Bvh4QNodeHost> quantize_bvh4(const Bvh4& b4)
     }
     return out;
 }
+
+float qbvh_plane_bound(const std::vector<Bvh4QNodeHost>& q4)
+{
+    double c = 0.0;
+    for (const Bvh4QNodeHost& q : q4) {
+        const double step[3] = {q.step_x, q.step_y, q.step_z};
+        for (int a = 0; a < 3; ++a) c = std::max(c, std::fabs(double(q.origin[a])) + 255.0 * step[a]);
+    }
+    return std::nextafter(float(c * (1.0 + 1e-6)), std::numeric_limits<float>::infinity());
+}

@@ -150,7 +150,9 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 // ---- ray/box and watertight ray/triangle -----------------------------------------------------
 struct RayPre {
     f3 o, d, inv;
-    f3 o_rot;  // origin components in (kx, ky, kz) order
+    f3 c;       // -(o * inv): slab offset of the quantized 4-wide BVH's node test
+    float k;    // slab margin of that test per unit |inv|
+    f3 o_rot;   // origin components in (kx, ky, kz) order
     int rot;   // = kx: offset of the (kx, ky, kz) run in a rotated triangle vertex (TriRot)
     float Sx, Sy, Sz;
 };
@@ -162,12 +164,25 @@ __device__ __forceinline__ float safe_rcp(float d)
     return 1.0f / (fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d);
 }
 
-__device__ __forceinline__ RayPre ray_pre(f3 o, f3 d)
+// Offset and margin of the quantized node test. A plane at P = origin + q * step (one rounding
+// on the host) is entered at (P - o) * inv; the node test evaluates it as fma(q, step * inv, b)
+// with b = fma(origin, inv, c), c = -(o * inv), widened to b -+ k |inv| (one fused multiply-add
+// each; entry planes down, exit planes up), so a plane costs one operation and an axis three more
+// per node. The roundings, each at most u = 2^-24 of its result, are: the host plane
+// u (|origin| + 255 step) |inv|, o * inv u |o| |inv| (the negation is exact), b u |b| with
+// |b| <= (|origin| + |o|) |inv| (1 + u), b -+ m u (|b| + m) and the plane distance
+// u (|b| + 255 step |inv| + m), m = k |inv|. With C >= |origin| + 255 step for every node
+// (plane_bound) they sum to at most u |inv| (4 C + 4 |o|) + 2 u m, which m exceeds for
+// k = 6e-7 (C + max |o|) (about 10 u (C + |o|)): the entry plane is never late and the exit plane
+// never early.
+__device__ __forceinline__ RayPre ray_pre(f3 o, f3 d, float plane_bound)
 {
     RayPre r;
     r.o = o;
     r.d = d;
     r.inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+    r.c = mk(-(o.x * r.inv.x), -(o.y * r.inv.y), -(o.z * r.inv.z));
+    r.k = 6e-7f * (plane_bound + fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)));
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
     int kx = kz + 1 == 3 ? 0 : kz + 1;
@@ -441,19 +456,15 @@ __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, co
         cna[k] = c == int(kQEmptyDev) ? -1 : c;
     }
     // Slab distances in ray space: (origin + q * step - o) * inv = q * a + b with a = step * inv
-    // (exact: step is a power of two) and b = (origin - o) * inv, so each plane costs one fused
+    // (exact: step is a power of two) and b = origin * inv - o * inv, so each plane costs one fused
     // multiply-add. The entry plane of an axis (the lower one for inv >= 0) is evaluated with
-    // b - e and the exit plane with b + e, e = 3e-7 ((255 step + |origin|) |inv| + |b|), which
-    // bounds the error against the distance of the plane as the host rounds it
-    // (origin + q * step, one rounding): 1.2e-7 |b| (b's two roundings), 6e-8 (255 step +
-    // |origin|) |inv| (the plane's rounding), 6e-8 (|b| + e) (rounding b -+ e) and 6e-8 |t| (the
-    // fused rounding), with |t| <= |b| + 255 step |inv| + e. Entry and exit need no min / max.
+    // b - k |inv| and the exit plane with b + k |inv|, a margin that bounds every rounding against
+    // the plane as the host rounds it (ray_pre). Entry and exit need no min / max.
     const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
-    const float bx = (ox - r.o.x) * r.inv.x, by = (oy - r.o.y) * r.inv.y, bz = (oz - r.o.z) * r.inv.z;
-    const float ex = 3e-7f * fmaf(fmaf(255.0f, sx, fabsf(ox)), fabsf(r.inv.x), fabsf(bx));
-    const float ey = 3e-7f * fmaf(fmaf(255.0f, sy, fabsf(oy)), fabsf(r.inv.y), fabsf(by));
-    const float ez = 3e-7f * fmaf(fmaf(255.0f, sz, fabsf(oz)), fabsf(r.inv.z), fabsf(bz));
-    const float bx0 = bx - ex, bx1 = bx + ex, by0 = by - ey, by1 = by + ey, bz0 = bz - ez, bz1 = bz + ez;
+    const float bx = fmaf(ox, r.inv.x, r.c.x), by = fmaf(oy, r.inv.y, r.c.y), bz = fmaf(oz, r.inv.z, r.c.z);
+    const float bx0 = fmaf(-r.k, fabsf(r.inv.x), bx), bx1 = fmaf(r.k, fabsf(r.inv.x), bx);
+    const float by0 = fmaf(-r.k, fabsf(r.inv.y), by), by1 = fmaf(r.k, fabsf(r.inv.y), by);
+    const float bz0 = fmaf(-r.k, fabsf(r.inv.z), bz), bz1 = fmaf(r.k, fabsf(r.inv.z), bz);
     const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
     const uint32_t nx = px ? w1.x : w1.y, fx = px ? w1.y : w1.x;  // entry / exit plane bytes
     const uint32_t ny = py ? w1.z : w1.w, fy = py ? w1.w : w1.z;
@@ -573,13 +584,12 @@ __device__ __forceinline__ bool trace4_node_pp(const float4* __restrict__ nodes,
         const int c = int(ref[k] >> 29);
         cna[k] = c == int(kQEmptyDev) ? -1 : c;
     }
-    // the slab test of trace4_step (same planes, same error bound)
+    // the slab test of trace4_step (same planes, same margins)
     const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
-    const float bx = (ox - r.o.x) * r.inv.x, by = (oy - r.o.y) * r.inv.y, bz = (oz - r.o.z) * r.inv.z;
-    const float ex = 3e-7f * fmaf(fmaf(255.0f, sx, fabsf(ox)), fabsf(r.inv.x), fabsf(bx));
-    const float ey = 3e-7f * fmaf(fmaf(255.0f, sy, fabsf(oy)), fabsf(r.inv.y), fabsf(by));
-    const float ez = 3e-7f * fmaf(fmaf(255.0f, sz, fabsf(oz)), fabsf(r.inv.z), fabsf(bz));
-    const float bx0 = bx - ex, bx1 = bx + ex, by0 = by - ey, by1 = by + ey, bz0 = bz - ez, bz1 = bz + ez;
+    const float bx = fmaf(ox, r.inv.x, r.c.x), by = fmaf(oy, r.inv.y, r.c.y), bz = fmaf(oz, r.inv.z, r.c.z);
+    const float bx0 = fmaf(-r.k, fabsf(r.inv.x), bx), bx1 = fmaf(r.k, fabsf(r.inv.x), bx);
+    const float by0 = fmaf(-r.k, fabsf(r.inv.y), by), by1 = fmaf(r.k, fabsf(r.inv.y), by);
+    const float bz0 = fmaf(-r.k, fabsf(r.inv.z), bz), bz1 = fmaf(r.k, fabsf(r.inv.z), bz);
     const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
     const uint32_t nx = px ? w1.x : w1.y, fx = px ? w1.y : w1.x;
     const uint32_t ny = py ? w1.z : w1.w, fy = py ? w1.w : w1.z;

@@ -98,6 +98,9 @@ static_assert(sizeof(Bvh4QNodeHost) == 64, "quantized node layout");
 constexpr uint32_t kQEmpty = 7u;
 
 std::vector<Bvh4QNodeHost> quantize_bvh4(const Bvh4& b4);
+// Largest |origin| + 255 * step over the axes of the nodes (rounded up): bounds the magnitude of
+// every plane of the quantized BVH, which the traversal's per-ray slab margins are sized by.
+float qbvh_plane_bound(const std::vector<Bvh4QNodeHost>& q4);
 
 // ---- lights ------------------------------------------------------------------------------
 enum : int32_t { kLightPoint = 0, kLightRect = 1 };

@@ -103,6 +103,7 @@ struct SceneDev {
     float3 sigma_s, sigma_t;
     float lambda;              // mean extinction (HomogeneousMedium.h:50-51)
     float medium_albedo;       // mean of sigma_s / sigma_t
+    float plane_bound;         // largest |plane| of the quantized BVH (qbvh_plane_bound)
 };
 
 struct RenderDev {
@@ -407,14 +408,16 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
                 rec = i;
                 const float4 d = Wv.ray_d[cur][i];
                 float3 c = Wv.cpos;
+                float pb = S0.plane_bound;
                 if constexpr (BATCH) {
                     const JobDev& J = jobs[__float_as_uint(d.w)];
                     nodes = J.S.nodes4;
                     tris = J.S.tris;
                     c = J.R.cpos;
+                    pb = J.S.plane_bound;
                 }
                 const float4 o = first ? make_float4(c.x, c.y, c.z, 0.0f) : Wv.ray_o[cur][i];
-                r = ray_pre(mk(o), mk(d));
+                r = ray_pre(mk(o), mk(d), pb);
                 trav4_begin(st, __int_as_float(0x7f800000));
                 blocked = false;
             });
@@ -447,14 +450,16 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
                 rec = i;
                 const float4 d = Wv.ray_d[cur][i];
                 float3 c = Wv.cpos;
+                float pb = S0.plane_bound;
                 if constexpr (BATCH) {
                     const JobDev& J = jobs[__float_as_uint(d.w)];
                     nodes = J.S.nodes4;
                     tris = J.S.tris;
                     c = J.R.cpos;
+                    pb = J.S.plane_bound;
                 }
                 const float4 o = first ? make_float4(c.x, c.y, c.z, 0.0f) : Wv.ray_o[cur][i];
-                r = ray_pre(mk(o), mk(d));
+                r = ray_pre(mk(o), mk(d), pb);
                 trav4_begin(st, __int_as_float(0x7f800000));
             });
             if (__ballot(busy) == 0ull) break;  // the queue is drained and every query complete
@@ -480,7 +485,7 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
                     const SceneDev& S = BATCH ? jobs[__float_as_uint(d.w)].S : S0;
                     const float3 c = BATCH ? jobs[__float_as_uint(d.w)].R.cpos : Wv.cpos;
                     const float4 o = first ? make_float4(c.x, c.y, c.z, 0.0f) : Wv.ray_o[cur][i];
-                    RayPre r = ray_pre(mk(o), mk(d));
+                    RayPre r = ray_pre(mk(o), mk(d), S.plane_bound);
                     float tfar = __int_as_float(0x7f800000);
                     int32_t tri = trace_scene<false, TRAV>(S, r, kSigmaVisibility, tfar, stack);
                     Wv.hit[i] = make_float2(tri >= 0 ? tfar : -1.0f, __int_as_float(tri));
@@ -1019,13 +1024,15 @@ __global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, con
         for (;;) {
             rf.run(count, chunk, Wv.counters, P_FETCH_SHADOW, busy, [&](uint32_t i) {
                 rec = i;
+                float pb = S0.plane_bound;
                 if constexpr (BATCH) {
                     const SceneDev& S = jobs[Wv.sh_job[i]].S;
                     nodes = S.nodes4;
                     tris = S.tris;
+                    pb = S.plane_bound;
                 }
                 const float4 o = Wv.sh_o[i], d = Wv.sh_d[i];
-                r = ray_pre(mk(o), mk(d));
+                r = ray_pre(mk(o), mk(d), pb);
                 trav4_begin(st, o.w - kSigmaVisibility);
             });
             if (__ballot(busy) == 0ull) break;
@@ -1049,7 +1056,7 @@ __global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, con
                 if (i < count) {
                     const SceneDev& S = BATCH ? jobs[Wv.sh_job[i]].S : S0;
                     float4 o = Wv.sh_o[i], d = Wv.sh_d[i];
-                    RayPre r = ray_pre(mk(o), mk(d));
+                    RayPre r = ray_pre(mk(o), mk(d), S.plane_bound);
                     float tfar = o.w - kSigmaVisibility;
                     bool vis = trace_scene<true, TRAV>(S, r, kSigmaVisibility, tfar, stack) < 0;
                     Wv.sh_vis[i] = vis ? 1 : 0;
@@ -1280,7 +1287,7 @@ __global__ void __launch_bounds__(kTraceBlock) intersect_kernel(SceneDev S, cons
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         tr_ray q = rays[i];
-        RayPre r = ray_pre(mk(q.org[0], q.org[1], q.org[2]), mk(q.dir[0], q.dir[1], q.dir[2]));
+        RayPre r = ray_pre(mk(q.org[0], q.org[1], q.org[2]), mk(q.dir[0], q.dir[1], q.dir[2]), S.plane_bound);
         float tfar = q.tfar;
         int32_t tri = trace_scene<false, TRAV>(S, r, q.tnear, tfar, stack);
         tr_hit h;
@@ -1309,7 +1316,7 @@ __global__ void __launch_bounds__(kTraceBlock) occluded_kernel(SceneDev S, const
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         tr_ray q = rays[i];
-        RayPre r = ray_pre(mk(q.org[0], q.org[1], q.org[2]), mk(q.dir[0], q.dir[1], q.dir[2]));
+        RayPre r = ray_pre(mk(q.org[0], q.org[1], q.org[2]), mk(q.dir[0], q.dir[1], q.dir[2]), S.plane_bound);
         float tfar = q.tfar;
         occluded[i] = trace_scene<true, TRAV>(S, r, q.tnear, tfar, stack) >= 0 ? 1 : 0;
     }
@@ -1591,6 +1598,7 @@ struct tr_scene_s {
     int depth = 0, depth4 = 0, stack4 = 0;
     bool wide = true;   // traverse the 4-wide BVH (tr_render_desc flags can select the binary one)
     float bb_lo[3] = {0, 0, 0}, bb_hi[3] = {1, 1, 1};  // bounds of the triangles (set at commit)
+    float plane_bound = 1.0f;  // largest |plane| of the quantized BVH (set at commit)
     LaunchCfg tune;     // launch shape used by renders of this scene (tr_autotune sets it)
     double build_ms = 0.0;
     uint64_t dev_bytes = 0;
@@ -1622,6 +1630,7 @@ struct tr_scene_s {
         s.sigma_t = make_float3(sa[0] + ss[0], sa[1] + ss[1], sa[2] + ss[2]);
         s.lambda = (s.sigma_t.x + s.sigma_t.y + s.sigma_t.z) / 3.0f;
         s.medium_albedo = has_medium ? (ss[0] / s.sigma_t.x + ss[1] / s.sigma_t.y + ss[2] / s.sigma_t.z) / 3.0f : 0.f;
+        s.plane_bound = plane_bound;
         return s;
     }
 };
@@ -1913,6 +1922,7 @@ int tr_scene_commit(tr_scene s)
     s->n_nodes4 = uint32_t(b4.nodes.size());
     s->depth4 = b4.depth;
     s->stack4 = b4.stack_bound;
+    s->plane_bound = qbvh_plane_bound(q4);
     s->build_ms = b.build_ms;
     s->committed = true;
     return TR_OK;
