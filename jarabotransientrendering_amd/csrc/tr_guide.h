@@ -76,13 +76,16 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef short s16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ uint32_t bf16_bits(float x)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// float -> bf16 bits, rounded to nearest even by the hardware conversion (v_cvt_pk_bf16_f32, two
+// values per instruction): the same bits as bf16_bits_host for every finite input
+__device__ __forceinline__ uint32_t bf16_bits(float x) { return __builtin_bit_cast(uint16_t, __bf16(x)); }
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi)
 {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return u >> 16;
+    const bf16x2 v = {__bf16(lo), __bf16(hi)};
+    return __builtin_bit_cast(uint32_t, v);
 }
-__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
 __device__ __forceinline__ float bf16_to_f32(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
 
 __device__ __forceinline__ f32x4 mfma32(uint4 a, uint4 b, f32x4 c)
