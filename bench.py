@@ -191,7 +191,7 @@ GUIDED_WORKLOADS = {
 
 def guided_comparison(name="icosphere_media", ref_sqrt=32, ref_spp=262144, pass_sqrt=16, checkpoints=(4, 8, 16, 32, 64),
                       train_passes=8, train_sqrt=8, steps_per_wave=64, train_paths=1 << 18, bsdf_fraction=0.3,
-                      learning_rate=3e-3, repeats=3):
+                      learning_rate=3e-3, batch_size=1 << 14, repeats=3):
     """Guided versus plain rendering of the same film, timed. The guide first trains on
     `train_passes` passes of train_sqrt^2 spp (one wavefront each, the first `train_paths` paths of
     which record training data, `steps_per_wave` Adam steps after
@@ -254,7 +254,8 @@ def guided_comparison(name="icosphere_media", ref_sqrt=32, ref_spp=262144, pass_
     guided_runs, infos = [], []
     for rep in range(repeats):
         guide = renderer.PathGuide(steps_per_wave=steps_per_wave, bsdf_fraction=bsdf_fraction,
-                                   learning_rate=learning_rate, train_paths=train_paths, init_seed=1 + rep)
+                                   learning_rate=learning_rate, train_paths=train_paths, batch_size=batch_size,
+                                   init_seed=1 + rep)
         guided_runs.append(run(guide, rep))
         infos.append(guide.info())
 
@@ -292,7 +293,8 @@ def guided_comparison(name="icosphere_media", ref_sqrt=32, ref_spp=262144, pass_
         "reference": {"spp": n_ref * ref_sqrt ** 2, "seconds": ref_sec, "estimator": "plain, orthonormal frame"},
         "rel_mse_eps": "1e-2 * mean(reference)^2",
         "guide_settings": {"bsdf_fraction": bsdf_fraction, "learning_rate": learning_rate,
-                           "steps_per_wave": steps_per_wave, "train_passes": train_passes, "train_paths": train_paths,
+                           "steps_per_wave": steps_per_wave, "batch_size": batch_size, "train_passes": train_passes,
+                           "train_paths": train_paths,
                            "train_pass_spp": train_sqrt ** 2},
         "repeats": repeats,
         "error_ratio_at_equal_spp": ratios,

@@ -1116,20 +1116,27 @@ __device__ __forceinline__ void film_add_partial(float* __restrict__ sum, float*
 // kFilmRange consecutive shadow records; their pixels are few (paths are enumerated pixel-major
 // and queue compaction keeps runs of neighbouring paths together), so each block accumulates into
 // LDS histograms of up to `pix_slots` pixels x T bins x RGB and flushes each non-zero bin with one
-// compensated add; records of further pixels go straight to HBM. The LDS histograms hold FP64
-// partials (non-returning LDS atomic adds): a partial of at most kFilmRange float contributions is
-// exact to within 2^-44 of their magnitude sum, far inside the film's compensated FP32 bound, and
-// is flushed as the float pair (hi, lo) = (float(p), float(p - hi)). Short ranges (512 records)
-// measured best: 1.6x faster than 4096-record ranges on the 1024-bin films,
-// whose per-pixel records spread over many bins. The kernel waits on its loads and returning
-// atomics: 1024-thread blocks (two per CU, 32 wavefronts) ran it 12-16 % faster than 256-thread
-// blocks (three per CU, 12 wavefronts), and 72 KB of LDS per block (three 1024-bin pixels instead
-// of two, still two blocks per CU) 10-14 % faster again: fewer records of a range miss a slot.
+// compensated add. Records of pixels that find no slot try again after the flush, in a second
+// round with the slots free, and only those left after kFilmRounds rounds go straight to HBM: the
+// range's records come from a few append chunks of the shading kernel, each holding paths of one
+// or two pixels, and the 256-path chunks of guided shading put more pixels in a range than the
+// slots hold (the second round: film time -4 % on the 1024-spp icosphere frames, -7 % on guided
+// media passes).
+// The LDS histograms hold FP64 partials (non-returning LDS atomic adds): a partial of at most
+// kFilmRange float contributions is exact to within 2^-44 of their magnitude sum, far inside the
+// film's compensated FP32 bound, and is flushed as the float pair (hi, lo) = (float(p),
+// float(p - hi)). Short ranges (512 records) measured best: 1.6x faster than 4096-record ranges on
+// the 1024-bin films, whose per-pixel records spread over many bins. The kernel waits on its loads
+// and returning atomics: 1024-thread blocks (two per CU, 32 wavefronts) ran it 12-16 % faster than
+// 256-thread blocks (three per CU, 12 wavefronts), and 72 KB of LDS per block (three 1024-bin
+// pixels instead of two, still two blocks per CU) 10-14 % faster again.
 // (StreakCameraFilm::add_samples / add_temporal_sample, StreakCameraFilm.h:139-192, 280-297)
 constexpr int kFilmBlock = 1024;
 constexpr uint32_t kFilmRange = 512;
 constexpr uint32_t kFilmLdsBytes = 72 * 1024;  // FP64 histograms: pix_slots * T * 3 * 8 bytes (3 pixels of 1024 bins)
 constexpr uint32_t kFilmMaxSlots = 8;
+constexpr int kFilmRounds = 2;
+static_assert(kFilmRange <= uint32_t(kFilmBlock), "one record of a range per thread");
 
 template <bool GUIDED>
 __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev Wv, int sh, uint32_t pix_slots, float* contrib)
@@ -1139,52 +1146,63 @@ __global__ void __launch_bounds__(kFilmBlock) film_kernel(RenderDev R, WaveDev W
     const uint32_t count = Wv.counters[shword(sh)];
     if (blockIdx.x * kFilmRange >= count) return;  // no range for this block (the last bounces' launches)
     const uint32_t T = uint32_t(R.T), T3 = 3u * T, hsize = pix_slots * T3;
+    const int rounds = pix_slots ? kFilmRounds : 1;
     // the histograms start at zero; each flush reads and clears only the slots that were used
     for (uint32_t i = threadIdx.x; i < hsize; i += blockDim.x) hist[i] = 0.0;
     for (uint32_t start = blockIdx.x * kFilmRange; start < count; start += gridDim.x * kFilmRange) {
-        const uint32_t end = min(start + kFilmRange, count);
-        if (threadIdx.x < kFilmMaxSlots) slot_pix[threadIdx.x] = 0xFFFFFFFFu;
-        __syncthreads();
-        for (uint32_t r = start + threadIdx.x; r < end; r += blockDim.x) {
-            if (!Wv.sh_vis[r]) continue;
-            float4 c = Wv.sh_c[r];
+        // this thread's record of the range, if it is a visible contribution
+        const uint32_t r = start + threadIdx.x;
+        bool pending = false;
+        float4 c = {};
+        uint32_t pix = 0, bin = 0;
+        if (threadIdx.x < kFilmRange && r < count && Wv.sh_vis[r]) {
+            c = Wv.sh_c[r];
             if constexpr (GUIDED) {
                 // luminance of the contribution, credited to the training record of its vertex
                 uint32_t tag = Wv.sh_tag[r];
                 if (tag != 0xFFFFFFFFu) atomicAdd(contrib + tag, (c.x + c.y + c.z) * (1.0f / 3.0f));
             }
-            uint32_t pix = __float_as_uint(c.w);
-            uint32_t bin = __float_as_uint(Wv.sh_d[r].w) - pix * T;
-            int slot = -1;
-            for (uint32_t k = 0; k < pix_slots; ++k) {
-                uint32_t v = atomicCAS(&slot_pix[k], 0xFFFFFFFFu, pix);
-                if (v == 0xFFFFFFFFu || v == pix) {
-                    slot = int(k);
-                    break;
+            pix = __float_as_uint(c.w);
+            bin = __float_as_uint(Wv.sh_d[r].w) - pix * T;
+            pending = true;
+        }
+        for (int round = 0; round < rounds; ++round) {
+            if (threadIdx.x < kFilmMaxSlots) slot_pix[threadIdx.x] = 0xFFFFFFFFu;
+            if (!__syncthreads_or(pending)) break;
+            if (pending) {
+                int slot = -1;
+                for (uint32_t k = 0; k < pix_slots; ++k) {
+                    uint32_t v = atomicCAS(&slot_pix[k], 0xFFFFFFFFu, pix);
+                    if (v == 0xFFFFFFFFu || v == pix) {
+                        slot = int(k);
+                        break;
+                    }
+                }
+                if (slot >= 0) {
+                    double* h = hist + uint32_t(slot) * T3 + bin * 3u;
+                    atomicAdd(h + 0, double(c.x));
+                    atomicAdd(h + 1, double(c.y));
+                    atomicAdd(h + 2, double(c.z));
+                    pending = false;
+                } else if (round == rounds - 1) {
+                    film_add3(R.film, R.film_comp, (size_t(pix) * T + bin) * 3, c.x, c.y, c.z);
+                    pending = false;
                 }
             }
-            if (slot >= 0) {
-                double* h = hist + uint32_t(slot) * T3 + bin * 3u;
-                atomicAdd(h + 0, double(c.x));
-                atomicAdd(h + 1, double(c.y));
-                atomicAdd(h + 2, double(c.z));
-            } else {
-                film_add3(R.film, R.film_comp, (size_t(pix) * T + bin) * 3, c.x, c.y, c.z);
+            __syncthreads();
+            uint32_t used = 0;  // slots are claimed in order
+            while (used < pix_slots && slot_pix[used] != 0xFFFFFFFFu) ++used;
+            for (uint32_t i = threadIdx.x; i < used * T3; i += blockDim.x) {
+                const double v = hist[i];
+                if (v != 0.0) {
+                    const uint32_t k = i / T3;
+                    const float hi = float(v), lo = float(v - double(hi));
+                    film_add_partial(R.film, R.film_comp, size_t(slot_pix[k]) * T3 + (i - k * T3), hi, lo);
+                    hist[i] = 0.0;
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
-        uint32_t used = 0;  // slots are claimed in order
-        while (used < pix_slots && slot_pix[used] != 0xFFFFFFFFu) ++used;
-        for (uint32_t i = threadIdx.x; i < used * T3; i += blockDim.x) {
-            const double v = hist[i];
-            if (v != 0.0) {
-                const uint32_t k = i / T3;
-                const float hi = float(v), lo = float(v - double(hi));
-                film_add_partial(R.film, R.film_comp, size_t(slot_pix[k]) * T3 + (i - k * T3), hi, lo);
-                hist[i] = 0.0;
-            }
-        }
-        __syncthreads();
     }
 }
 
