@@ -100,15 +100,16 @@ __device__ __forceinline__ f32x4 mfma32(uint4 a, uint4 b, f32x4 c)
 //   t/total_time and sin/cos(2^f * pi * t) for f = 0..4                                11
 //   medium flag                                                                         1
 //   zero padding                                                                        3
-// sin(pi 2^q x), cos(pi 2^q x) for q = 0 .. Q-1 into f[2q], f[2q + 1]: one sincospi, then the
-// double-angle identities sin 2a = 2 sin a cos a, cos 2a = 1 - 2 sin^2 a (each doubling at most
-// doubles the absolute error, so the top octave is within ~2^Q ulp, far below the bf16 rounding
-// the network's inputs get)
+// sin(pi 2^q x), cos(pi 2^q x) for q = 0 .. Q-1 into f[2q], f[2q + 1]: the hardware sine and
+// cosine (v_sin_f32 / v_cos_f32 take their argument in turns; the fraction keeps it inside their
+// valid range), then the double-angle identities sin 2a = 2 sin a cos a, cos 2a = 1 - 2 sin^2 a
+// (each doubling at most doubles the absolute error: the top octave stays within ~1e-5, far below
+// the bf16 rounding the network's inputs get)
 template <int Q>
 __device__ __forceinline__ void octaves(float x, float* f)
 {
-    float sn, cs;
-    sincospif(x, &sn, &cs);
+    const float turns = __builtin_amdgcn_fractf(0.5f * x);
+    float sn = __builtin_amdgcn_sinf(turns), cs = __builtin_amdgcn_cosf(turns);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         f[2 * q] = sn;
@@ -243,15 +244,20 @@ struct Lobe {
 };
 
 __device__ __forceinline__ float softplus(float x) { return x > 15.f ? x : log1pf(__expf(x)); }
+// the same from the hardware exp / log (shading: kappa is a parameter of the proposal that is both
+// sampled and evaluated, so its last digits need not match the training kernel's)
+__device__ __forceinline__ float softplus_fast(float x) { return x > 15.f ? x : __logf(1.0f + __expf(x)); }
 
 constexpr float kKappaMin = 1e-2f;
 constexpr float kKappaMax = 2e3f;
 
 // The mixture of one sample, decoded once from its 40 raw outputs (kept in LDS): per lobe the unit
-// mean direction, kappa, the vMF density's normaliser kappa / (2 pi (1 - e^{-2 kappa})) (through
-// expm1, stable for small kappa; the density is norm * exp(kappa (mu.w - 1))) and the softmax
-// weight, in registers, so that picking a lobe, sampling it and evaluating the mixture density
-// decode nothing again.
+// mean direction, kappa, the vMF density's normaliser kappa / (2 pi (1 - e^{-2 kappa})) (the
+// density is norm * exp(kappa (mu.w - 1))) and the softmax weight, in registers, so that picking a
+// lobe, sampling it and evaluating the mixture density decode nothing again. The decode uses the
+// hardware exp, log and reciprocal (about 1 ulp): with kappa >= 0.01, 1 - e^{-2 kappa} >= 0.0198
+// keeps the normaliser within ~1e-5 relative without expm1, and the density that weights a sample
+// is the one it was drawn from up to that.
 struct Mixture {
     float mx[kLobes], my[kLobes], mz[kLobes], kappa[kLobes], norm[kLobes], w[kLobes];
 };
@@ -266,7 +272,7 @@ __device__ __forceinline__ void mixture_decode(const float* o, Mixture& M)
         M.w[i] = __expf(o[5 * i + 4] - lmax);
         sum += M.w[i];
     }
-    const float inv_sum = 1.0f / sum;
+    const float inv_sum = __builtin_amdgcn_rcpf(sum);
 #pragma unroll
     for (int i = 0; i < kLobes; ++i) {
         M.w[i] *= inv_sum;
@@ -278,9 +284,9 @@ __device__ __forceinline__ void mixture_decode(const float* o, Mixture& M)
         } else {
             M.mx[i] = 0.f, M.my[i] = 0.f, M.mz[i] = 1.f;
         }
-        const float k = fminf(softplus(o[5 * i + 3]) + kKappaMin, kKappaMax);
+        const float k = fminf(softplus_fast(o[5 * i + 3]) + kKappaMin, kKappaMax);
         M.kappa[i] = k;
-        M.norm[i] = k / (2.0f * 3.14159265358979f * -expm1f(-2.0f * k));
+        M.norm[i] = k * __builtin_amdgcn_rcpf(2.0f * 3.14159265358979f * (1.0f - __expf(-2.0f * k)));
     }
 }
 // lobe i of the mixture (i uniform or not: selected without indexing the register arrays)
@@ -320,17 +326,17 @@ __device__ __forceinline__ int pick_lobe(const Mixture& M, float u)
 }
 
 // Sample a vMF lobe (W. Jakob, "Numerically stable sampling of the von Mises-Fisher distribution
-// on S^2", 2012): cos = 1 + log(u1 + (1 - u1) e^{-2 kappa}) / kappa, azimuth 2 pi u2.
+// on S^2", 2012): cos = 1 + log(u1 + (1 - u1) e^{-2 kappa}) / kappa, azimuth 2 pi u2 (hardware
+// log, reciprocal, and sine / cosine, whose argument is in turns: u2 itself).
 __device__ __forceinline__ void vmf_sample(const Lobe& l, float u1, float u2, float* wx, float* wy, float* wz)
 {
-    float ct = 1.0f + logf(u1 + (1.0f - u1) * __expf(-2.0f * l.kappa)) / l.kappa;
+    float ct = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * l.kappa)) * __builtin_amdgcn_rcpf(l.kappa);
     ct = fminf(fmaxf(ct, -1.0f), 1.0f);
     float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
-    float sp, cp;
-    sincospif(2.0f * u2, &sp, &cp);
-    // orthonormal basis around mu (Duff et al. 2017)
+    const float sp = __builtin_amdgcn_sinf(u2), cp = __builtin_amdgcn_cosf(u2);
+    // orthonormal basis around mu (Duff et al. 2017); |sign + mu_z| >= 1
     float sign = copysignf(1.0f, l.mz);
-    float a = -1.0f / (sign + l.mz);
+    float a = -__builtin_amdgcn_rcpf(sign + l.mz);
     float b = l.mx * l.my * a;
     float tx = 1.0f + sign * l.mx * l.mx * a, ty = sign * b, tz = -sign * l.mx;
     float bx = b, by = sign + l.my * l.my * a, bz = -l.my;
