@@ -140,8 +140,10 @@ __device__ __forceinline__ void encode_features(const float* raw, float* f)
 }
 
 // Per-wavefront LDS block of the network: 64 rows of kRow words. Row s first holds sample s's
-// 64 bf16 features (32 words, 16 zero columns), later its 40 FP32 outputs.
-constexpr int kRow = kOut;
+// 64 bf16 features (32 words, 16 zero columns), later its 40 FP32 outputs. The row stride is odd
+// (41 words) so that the 64 lanes reading word k of their own rows hit 64 different banks (a
+// stride of 40 words put them on 8 banks: 8-way conflicts on every read of the outputs).
+constexpr int kRow = kOut + 1;
 
 __device__ __forceinline__ void store_feature_row(uint32_t* row32, const float* f)
 {
@@ -154,7 +156,7 @@ __device__ __forceinline__ void store_feature_row(uint32_t* row32, const float* 
 // One wavefront evaluates the network for 64 samples held in `buf` (64 x kRow words, see above),
 // NST tiles of 16 samples per pass (more tiles: fewer weight-fragment reads, more registers).
 // `frag` holds the forward fragments (64 lanes x uint4 each; LDS or global memory), `bias` the
-// padded biases (64 per layer). The 40 outputs of sample s replace its features in row s; a row
+// padded biases (64 per layer; LDS or global memory). The 40 outputs of sample s replace its features in row s; a row
 // is overwritten only after its pass has read it. Every lane of the wavefront must call this.
 // If `acts` is not null (training), the neuron-major bf16 inputs of layers 1..4 are also stored:
 // acts[(l - 1) * 64 * 64 + n * 64 + s].
@@ -172,10 +174,9 @@ __device__ __forceinline__ void mlp_forward_wave(float* buf, const uint4* frag, 
         for (int t = 0; t < NST; ++t)
 #pragma unroll
             for (int kp = 0; kp < 2; ++kp) {
+                // 32-bit loads: rows of an odd word count are only 4-byte aligned
                 const uint32_t* r = rows + ((st0 + t) * 16 + c) * kRow + kp * 16;
-                uint2 lo = *reinterpret_cast<const uint2*>(r + 2 * g);
-                uint2 hi = *reinterpret_cast<const uint2*>(r + 8 + 2 * g);
-                B[t][kp] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+                B[t][kp] = make_uint4(r[2 * g], r[2 * g + 1], r[8 + 2 * g], r[9 + 2 * g]);
             }
         __builtin_amdgcn_wave_barrier();
 #pragma unroll

@@ -91,11 +91,11 @@ __global__ void __launch_bounds__(kEvalBlock) eval_kernel(const float* __restric
                                                           const uint4* __restrict__ frag,
                                                           const float* __restrict__ bias, float* __restrict__ out)
 {
-    __shared__ float lds[kEvalBlock / 64][64 * kRow];  // features (bf16 rows) then outputs, per wavefront
+    // features (bf16 rows) then outputs per wavefront, and the weight fragments: 80.9 KB, two
+    // blocks per CU (the biases are read from global memory, where they stay in the L1 cache)
+    __shared__ float lds[kEvalBlock / 64][64 * kRow];
     __shared__ uint4 wfrag[kFwdFrags * 64];
-    __shared__ float wbias[kBiasPad];
     for (int k = threadIdx.x; k < kFwdFrags * 64; k += kEvalBlock) wfrag[k] = frag[k];
-    for (int k = threadIdx.x; k < kBiasPad; k += kEvalBlock) wbias[k] = bias[k];
     __syncthreads();
     const int lane = int(__lane_id()), wid = int(threadIdx.x >> 6);
     float* mine = lds[wid];
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kEvalBlock) eval_kernel(const float* __restric
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        mlp_forward_wave<NST>(mine, wfrag, wbias, nullptr);
+        mlp_forward_wave<NST>(mine, wfrag, bias, nullptr);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -238,7 +238,7 @@ __device__ __forceinline__ float train_tile(const TrainArgs& A, uint32_t r0, uin
     // output gradient, one record per lane
     {
         float o[kOut], d[kOut];
-        for (int k = 0; k < kOut; ++k) o[k] = U[lane * kOut + k];
+        for (int k = 0; k < kOut; ++k) o[k] = U[lane * kRow + k];
         __syncthreads();  // every output read before dY (the same LDS) is written
         float loss = 0.f;
         if (w > 0.f) {
@@ -319,7 +319,7 @@ __global__ void __launch_bounds__(64) train_kernel(TrainArgs A)
     // U (feature rows, then the raw outputs) and dY share their LDS: the outputs are read into
     // registers before the first output gradient is written, so a block needs 50 KB instead of
     // 58 KB (three blocks per CU instead of two)
-    __shared__ float U[64 * kOut];
+    __shared__ float U[64 * kRow];
     static_assert(64 * 64 * sizeof(uint16_t) <= sizeof(U), "dY fits in U");
     uint16_t* dY = reinterpret_cast<uint16_t*>(U);
     const uint32_t total = *A.count;

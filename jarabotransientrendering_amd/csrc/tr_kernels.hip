@@ -674,14 +674,13 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
     __shared__ uint32_t scan[kShadeBlock / 64 + 2];
     __shared__ float glds[GUIDED ? (kShadeBlockGuided / 64) * 64 * trg::kRow : 1];
     __shared__ uint4 gfrag[GUIDED ? trg::kFwdFrags * 64 : 1];
-    __shared__ float gbias[GUIDED ? trg::kBiasPad : 1];
     const uint32_t count = Wv.counters[qword(cur)];
     if (blockIdx.x == 0 && threadIdx.x < kParts) Wv.counters[part_word(threadIdx.x, P_FETCH_EXT)] = 0;  // for the next extend
     if (blockIdx.x * blockDim.x >= count) return;  // no paths for this block (the last bounces' launches)
     if constexpr (GUIDED) {
-        // the network's weights stay in LDS for the whole block (38 KiB + biases)
+        // the network's weight fragments stay in LDS for the whole block (38 KiB; with the rows of
+        // four wavefronts 80.3 KB, two blocks per CU); the biases are read from global memory
         for (uint32_t k = threadIdx.x; k < uint32_t(trg::kFwdFrags * 64); k += blockDim.x) gfrag[k] = G.frag[k];
-        for (uint32_t k = threadIdx.x; k < uint32_t(trg::kBiasPad); k += blockDim.x) gbias[k] = G.bias[k];
         __syncthreads();
     }
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -880,7 +879,7 @@ __global__ void __launch_bounds__(MODE == kShadeGuided ? kShadeBlockGuided : kSh
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                trg::mlp_forward_wave<1>(mine, gfrag, gbias, nullptr);
+                trg::mlp_forward_wave<1>(mine, gfrag, G.bias, nullptr);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
