@@ -159,15 +159,17 @@ __device__ __forceinline__ void store_feature_row(uint32_t* row32, const float* 
 // padded biases (64 per layer; LDS or global memory). The 40 outputs of sample s replace its features in row s; a row
 // is overwritten only after its pass has read it. Every lane of the wavefront must call this.
 // If `acts` is not null (training), the neuron-major bf16 inputs of layers 1..4 are also stored:
-// acts[(l - 1) * 64 * 64 + n * 64 + s].
+// acts[(l - 1) * 64 * 64 + n * 64 + s]. Only the sample tiles [st_begin, st_end) are evaluated
+// (the training kernel splits a tile's four sample tiles over four wavefronts).
 template <int NST>
-__device__ __forceinline__ void mlp_forward_wave(float* buf, const uint4* frag, const float* bias, uint16_t* acts)
+__device__ __forceinline__ void mlp_forward_wave(float* buf, const uint4* frag, const float* bias, uint16_t* acts,
+                                                 int st_begin = 0, int st_end = 4)
 {
     const int lane = int(__lane_id());
     const int g = lane >> 4, c = lane & 15;
     const uint32_t* rows = reinterpret_cast<const uint32_t*>(buf);
 #pragma unroll 1
-    for (int st0 = 0; st0 < 4; st0 += NST) {
+    for (int st0 = st_begin; st0 < st_end; st0 += NST) {
         // B operands of layer 0 straight from the feature rows: sample (st*16 + c), inputs k(g, j)
         uint4 B[NST][2];
 #pragma unroll

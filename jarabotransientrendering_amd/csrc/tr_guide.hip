@@ -183,18 +183,26 @@ __device__ float loss_grad(const float* o, float wx, float wy, float wz, float w
     return -w * log_q;
 }
 
-// ---- training: one wavefront per block, tiles of 64 records ------------------------------------
+// ---- training: tiles of 64 records, four wavefronts per tile -----------------------------------
 // LDS: acts[5][64][64] bf16 neuron-major layer inputs (40 KiB), U: feature rows / raw outputs
-// (10 KiB), and dY[64][64] bf16 neuron-major (8 KiB) in U's memory once the outputs are read:
-// 50 KiB per block, so three blocks fit on a CU (with 58 KiB, two did: the kernel waits on LDS and
-// memory more than it computes, and the third resident wavefront made it 21 % faster).
+// (10.25 KiB), and dY[64][64] bf16 neuron-major (8 KiB) in U's memory once the outputs are read:
+// 50.3 KiB per block, three blocks per CU.
+// A tile's work is split over the block's four wavefronts: wavefront w evaluates sample tile w of
+// the forward pass, owns output tile w of each layer's weight gradient, outputs 16 w .. 16 w + 15
+// of its bias gradient and input tile w of its input gradient; the per-record work (features, loss
+// gradient) runs on 16 lanes of each wavefront, records 16 w .. 16 w + 15. With one wavefront per
+// tile the kernel was bound by that wavefront's instruction issue (22,500 instructions per tile,
+// 10,000 of them scalar, one issue slot per 4 cycles: 85 us per step over 16,384 records, one
+// wavefront per CU).
 // Block b takes tiles b, b + gridDim.x, ... of the step's records and writes their gradient (sum
 // over the records) to partial[b][kParams] and their loss sum to partial_loss[b]; the block's first
-// tile stores, later ones add (each lane reads back only what it wrote). Launches use one block per
-// tile (train_grid): a grid of two blocks per CU that accumulated 8 tiles each ran 1.5x slower, its
-// read-add-write of the partial waiting on one dependent load per output tile. The weight-gradient
-// tiles come out of the matrix cores row-major in the parameter layout (W_l[i][o], lanes of a
-// 16-lane group on consecutive o), so every store writes 64-byte segments.
+// tile stores, later ones add (each lane reads back only what it wrote: the split above is the same
+// for every tile). Launches use one block per tile (train_grid). The weight-gradient tiles come out
+// of the matrix cores row-major in the parameter layout (W_l[i][o], lanes of a 16-lane group on
+// consecutive o), so every store writes 64-byte segments.
+constexpr int kTrainWaves = 4;
+constexpr int kTrainBlock = 64 * kTrainWaves;
+
 struct TrainArgs {
     const float4* rec;
     const uint32_t* count;   // records available (device)
@@ -209,14 +217,17 @@ struct TrainArgs {
 };
 
 // One 64-record tile of a training step: forward, loss gradient, backward; adds the tile's weight
-// and bias gradient into the block's partial P (stores when `add` is false) and returns its loss.
+// and bias gradient into the block's partial P (stores when `add` is false) and returns this
+// wavefront's share of the tile's loss.
 __device__ __forceinline__ float train_tile(const TrainArgs& A, uint32_t r0, uint32_t total, uint32_t avail, bool add,
                                             float* P, uint16_t (*acts)[64 * 64], uint16_t* dY, float* U)
 {
-    const int lane = int(__lane_id()), g = lane >> 4, c = lane & 15;
-    float tile_loss = 0.f;
-    const uint32_t ridx = r0 + uint32_t(lane);
-    const bool valid = ridx < avail;
+    const int lane = int(__lane_id()), g = lane >> 4, c = lane & 15, wid = int(threadIdx.x >> 6);
+    // this lane's record (lanes 0..15 of each wavefront)
+    const bool rec_lane = lane < 16;
+    const int rl = wid * 16 + c;
+    const uint32_t ridx = r0 + uint32_t(rl);
+    const bool valid = rec_lane && ridx < avail;
     float4 q0 = {}, q1 = {}, q2 = {}, q3 = {};
     if (valid) {
         const float4* r = A.rec + size_t(A.first + ridx) * 4;
@@ -225,41 +236,51 @@ __device__ __forceinline__ float train_tile(const TrainArgs& A, uint32_t r0, uin
     const float mean_w = *A.wsum / float(max(total, 1u));
     const float w = valid && mean_w > 0.f ? q2.w / mean_w : 0.0f;
     // features: sample-major rows for the forward pass, neuron-major copy for the weight gradient
-    {
+    if (rec_lane) {
         float raw[kRaw] = {q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, q0.w, q1.w, 0.f};
         float f[kFeat];
         encode_features(raw, f);
-        store_feature_row(reinterpret_cast<uint32_t*>(U) + lane * kRow, f);
-        for (int k = 0; k < 64; ++k) acts[0][k * 64 + lane] = k < kFeat ? uint16_t(bf16_bits(f[k])) : uint16_t(0);
+        store_feature_row(reinterpret_cast<uint32_t*>(U) + rl * kRow, f);
+#pragma unroll
+        for (int k = 0; k < 64; ++k) acts[0][k * 64 + rl] = k < kFeat ? uint16_t(bf16_bits(f[k])) : uint16_t(0);
     }
     __syncthreads();
-    mlp_forward_wave<4>(U, A.frag, A.bias, &acts[1][0]);
+    mlp_forward_wave<1>(U, A.frag, A.bias, &acts[1][0], wid, wid + 1);
     __syncthreads();
     // output gradient, one record per lane
+    float loss = 0.f;
     {
         float o[kOut], d[kOut];
-        for (int k = 0; k < kOut; ++k) o[k] = U[lane * kRow + k];
-        __syncthreads();  // every output read before dY (the same LDS) is written
-        float loss = 0.f;
-        if (w > 0.f) {
-            loss = loss_grad(o, q3.x, q3.y, q3.z, w, d);
-        } else {
-            for (int k = 0; k < kOut; ++k) d[k] = 0.f;
+        if (rec_lane) {
+#pragma unroll
+            for (int k = 0; k < kOut; ++k) o[k] = U[rl * kRow + k];
         }
-        for (int k = 0; k < 64; ++k) dY[k * 64 + lane] = k < kOut ? uint16_t(bf16_bits(d[k])) : uint16_t(0);
+        __syncthreads();  // every output read before dY (the same LDS) is written
+        if (rec_lane) {
+            if (w > 0.f) {
+                loss = loss_grad(o, q3.x, q3.y, q3.z, w, d);
+            } else {
+#pragma unroll
+                for (int k = 0; k < kOut; ++k) d[k] = 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < 64; ++k) dY[k * 64 + rl] = k < kOut ? uint16_t(bf16_bits(d[k])) : uint16_t(0);
+        }
         // wavefront loss sum
         for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-        tile_loss = loss;
     }
     __syncthreads();
     for (int l = kLayers - 1; l >= 0; --l) {
         const int n_in = kIn[l], n_out = kOutL[l];
         const uint16_t* X = acts[l];
-        // weight gradient D[i][o] = sum_s X[i][s] dY[o][s], stored as P[W_l + i*n_out + o]: the
-        // output tile's rows are inputs i (lane group g, element j), its columns outputs o (lane c)
-        for (int ot = 0; ot < (n_out + 15) / 16; ++ot)
+        // weight gradient D[i][o] = sum_s X[i][s] dY[o][s] of output tile ot = wid, stored as
+        // P[W_l + i*n_out + o]: the output tile's rows are inputs i (lane group g, element j), its
+        // columns outputs o (lane c)
+        if (wid * 16 < n_out) {
+            const int ot = wid;
             for (int it = 0; it < n_in / 16; ++it) {
                 f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
                 for (int ks = 0; ks < 2; ++ks) {
                     uint4 a = *reinterpret_cast<const uint4*>(X + (it * 16 + c) * 64 + ks * 32 + 8 * g);
                     uint4 b = *reinterpret_cast<const uint4*>(dY + (ot * 16 + c) * 64 + ks * 32 + 8 * g);
@@ -267,27 +288,40 @@ __device__ __forceinline__ float train_tile(const TrainArgs& A, uint32_t r0, uin
                 }
                 const int o = ot * 16 + c;
                 if (o < n_out)
+#pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         float* q = P + param_offset(l) + (it * 16 + 4 * g + j) * n_out + o;
                         *q = add ? *q + acc[j] : acc[j];
                     }
             }
-        // bias gradient
-        if (lane < n_out) {
+        }
+        // bias gradient of outputs 16 wid + c: lane group g sums samples 16 g .. 16 g + 15
+        {
+            const int o = wid * 16 + c;
             float sb = 0.f;
-            for (int s = 0; s < 64; ++s) sb += bf16_to_f32(dY[lane * 64 + s]);
-            float* q = P + param_offset(l) + n_in * n_out + lane;
-            *q = add ? *q + sb : sb;
+            if (o < n_out)
+#pragma unroll
+                for (int s = 0; s < 16; ++s) sb += bf16_to_f32(dY[o * 64 + g * 16 + s]);
+            sb += __shfl_xor(sb, 16);
+            sb += __shfl_xor(sb, 32);
+            if (g == 0 && o < n_out) {
+                float* q = P + param_offset(l) + n_in * n_out + o;
+                *q = add ? *q + sb : sb;
+            }
         }
         if (l == 0) break;
-        // input gradient dX[i][s] = sum_o W[i][o] dY[o][s], masked by ReLU'(layer input)
-        f32x4 acc[4][4];
-        for (int it = 0; it < 4; ++it)
-            for (int st = 0; st < 4; ++st) acc[it][st] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // input gradient dX[i][s] = sum_o W[i][o] dY[o][s] of input tile it = wid, masked by
+        // ReLU'(layer input)
+        f32x4 acc[4];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) acc[st] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
         for (int op = 0; op < 2; ++op) {
             uint4 B[4];
+#pragma unroll
             for (int st = 0; st < 4; ++st) {
                 uint32_t wv[4];
+#pragma unroll
                 for (int h = 0; h < 4; ++h) {
                     uint32_t lo = dY[(op * 32 + kperm(g, 2 * h)) * 64 + st * 16 + c];
                     uint32_t hi = dY[(op * 32 + kperm(g, 2 * h + 1)) * 64 + st * 16 + c];
@@ -295,31 +329,31 @@ __device__ __forceinline__ float train_tile(const TrainArgs& A, uint32_t r0, uin
                 }
                 B[st] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
             }
-            for (int it = 0; it < 4; ++it) {
-                uint4 a = A.bfrag[((l - 1) * kBwdFragsPerLayer + it * 2 + op) * 64 + lane];
-                for (int st = 0; st < 4; ++st) acc[it][st] = mfma32(a, B[st], acc[it][st]);
-            }
+            uint4 a = A.bfrag[((l - 1) * kBwdFragsPerLayer + wid * 2 + op) * 64 + lane];
+#pragma unroll
+            for (int st = 0; st < 4; ++st) acc[st] = mfma32(a, B[st], acc[st]);
         }
-        __syncthreads();
-        for (int it = 0; it < 4; ++it)
-            for (int st = 0; st < 4; ++st)
-                for (int j = 0; j < 4; ++j) {
-                    int i = it * 16 + 4 * g + j, s = st * 16 + c;
-                    float v = X[i * 64 + s] != 0 ? acc[it][st][j] : 0.0f;  // ReLU' of the layer input
-                    dY[i * 64 + s] = uint16_t(bf16_bits(v));
-                }
+        __syncthreads();  // every wavefront has read this layer's dY
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                int i = wid * 16 + 4 * g + j, s = st * 16 + c;
+                float v = X[i * 64 + s] != 0 ? acc[st][j] : 0.0f;  // ReLU' of the layer input
+                dY[i * 64 + s] = uint16_t(bf16_bits(v));
+            }
         __syncthreads();
     }
-    return tile_loss;
+    return loss;
 }
 
-__global__ void __launch_bounds__(64) train_kernel(TrainArgs A)
+__global__ void __launch_bounds__(kTrainBlock) train_kernel(TrainArgs A)
 {
     __shared__ uint16_t acts[kLayers][64 * 64];
     // U (feature rows, then the raw outputs) and dY share their LDS: the outputs are read into
-    // registers before the first output gradient is written, so a block needs 50 KB instead of
-    // 58 KB (three blocks per CU instead of two)
+    // registers before the first output gradient is written
     __shared__ float U[64 * kRow];
+    __shared__ float wave_loss[kTrainWaves];
     static_assert(64 * 64 * sizeof(uint16_t) <= sizeof(U), "dY fits in U");
     uint16_t* dY = reinterpret_cast<uint16_t*>(U);
     const uint32_t total = *A.count;
@@ -330,7 +364,13 @@ __global__ void __launch_bounds__(64) train_kernel(TrainArgs A)
         loss += train_tile(A, r0, total, avail, r0 != blockIdx.x * 64u, P, acts, dY, U);
         __syncthreads();  // the next tile overwrites the LDS arrays
     }
-    if (__lane_id() == 0 && blockIdx.x * 64u < avail) A.partial_loss[blockIdx.x] = loss;
+    if (__lane_id() == 0) wave_loss[threadIdx.x >> 6] = loss;
+    __syncthreads();
+    if (threadIdx.x == 0 && blockIdx.x * 64u < avail) {
+        float s = 0.f;
+        for (int k = 0; k < kTrainWaves; ++k) s += wave_loss[k];
+        A.partial_loss[blockIdx.x] = s;
+    }
 }
 
 // Blocks of a training step: one per tile of 64 records
@@ -555,7 +595,7 @@ struct tr_guide_s {
             T.partial = partial;
             T.partial_loss = partial_loss;
             const uint32_t tg = train_grid(desc.batch_size);
-            train_kernel<<<tg, 64, 0, s>>>(T);
+            train_kernel<<<tg, kTrainBlock, 0, s>>>(T);
             GHIP_TRY(hipGetLastError());
             ++steps;
             AdamArgs A;
@@ -797,7 +837,7 @@ int tr_guide_gradient(tr_guide g, const float* records, uint32_t n, float* grad,
     T.bfrag = reinterpret_cast<const uint4*>(g->bwd);
     T.bias = g->bias, T.partial = g->partial, T.partial_loss = g->partial_loss;
     const uint32_t tg = train_grid(n);
-    train_kernel<<<tg, 64, 0, g->stream>>>(T);
+    train_kernel<<<tg, kTrainBlock, 0, g->stream>>>(T);
     hipError_t e = hipGetLastError();
     std::vector<float> part(size_t(tg) * kParams), pl(tg);
     if (e == hipSuccess)
