@@ -181,8 +181,13 @@ GUIDED_WORKLOADS = {
                             train_passes=8, train_sqrt=8, steps_per_wave=64, train_paths=1 << 18),
     # the 1024-bin media streak film: one 512-pixel scanline of the medium-filled box, point light
     "media_scanline": dict(ref_sqrt=64, ref_spp=1 << 22, pass_sqrt=64, checkpoints=(4, 8, 16, 32, 64),
-                           train_passes=4, train_sqrt=32, steps_per_wave=32, train_paths=1 << 19, bsdf_fraction=0.5),
+                           train_passes=5, train_sqrt=32, steps_per_wave=64, train_paths=1 << 19, bsdf_fraction=0.5),
 }
+# Training schedules, swept with eight repeats each (median time-to-target speedup on the trimmed
+# per-bin relMSE): media streak film 4 x 32 Adam steps 1.25x, 4 x 48 1.42x (one repeat diverged),
+# 4 x 64 1.39x, 4 x 96 1.48x (worst repeat 1.28x), 5 x 64 1.42-1.47x (worst 1.35x), 6 x 64 1.48x
+# (worst 1.27x), 8 x 48 1.36x; icosphere 8 x 64 1.23x, 8 x 96 1.20x, 8 x 128 1.20x, 12 x 64 1.17x,
+# 16 x 32 1.16x. Batches of 65,536 records instead of 16,384 lost (0.64-0.78x at 4 x 8..32).
 # BSDF / phase-function fraction of the one-sample MIS mixture, swept over 0.3 / 0.5 / 0.7 with three
 # training repeats each: 0.3 gives the icosphere workload its best trimmed per-bin relMSE, 0.5 the
 # media streak film its best on every metric (a larger fraction also bounds each vertex's weight by
