@@ -223,14 +223,18 @@ __device__ __forceinline__ float train_tile(const TrainArgs& A, uint32_t r0, uin
                                             float* P, uint16_t (*acts)[64 * 64], uint16_t* dY, float* U)
 {
     const int lane = int(__lane_id()), g = lane >> 4, c = lane & 15, wid = int(threadIdx.x >> 6);
-    // this lane's record (lanes 0..15 of each wavefront)
+    // this lane's record (lanes 0..15 of each wavefront): record j = first + ridx of the step
+    // sequence is read at (j * p) mod total, p = 2^31 - 1 (a prime, so the map is a bijection of the
+    // total records): each step's batch is spread over the whole pass instead of a run of adjacent
+    // paths (neighbouring pixels), whatever order the batch was built in
     const bool rec_lane = lane < 16;
     const int rl = wid * 16 + c;
     const uint32_t ridx = r0 + uint32_t(rl);
     const bool valid = rec_lane && ridx < avail;
     float4 q0 = {}, q1 = {}, q2 = {}, q3 = {};
     if (valid) {
-        const float4* r = A.rec + size_t(A.first + ridx) * 4;
+        const uint32_t src = uint32_t((uint64_t(A.first + ridx) * 2147483647ull) % uint64_t(total));
+        const float4* r = A.rec + size_t(src) * 4;
         q0 = r[0], q1 = r[1], q2 = r[2], q3 = r[3];
     }
     const float mean_w = *A.wsum / float(max(total, 1u));
@@ -442,45 +446,63 @@ __global__ void adam_kernel(AdamArgs A)
     A.master[p] -= A.lr * mh / (sqrtf(vh) + A.eps);
 }
 
-// After a wave: records of training paths -> batch (weight = pre * sum of later contributions)
+// After a wave: records of training paths -> batch (weight = pre * sum of later contributions).
+// One thread per training path walks its levels from the last down, keeping the running sum of the
+// later levels' contributions, in two passes: the first counts the records it keeps (one block
+// atomic on the batch count per 256 paths), the second writes them; the weight sum is added once
+// per block. (A thread per record that re-summed the later levels with dependent loads took
+// 445 us per wave over 2^19 paths x 17 levels.) The batch's order is not the order the training
+// steps see: they visit it through a bijection (train_tile).
 __global__ void build_batch_kernel(trg::GuideView V, float4* __restrict__ batch, uint32_t* __restrict__ count,
                                    float* __restrict__ wsum)
 {
     __shared__ uint32_t scan[256 / 64 + 1];
     __shared__ float wred[256 / 64];
-    const uint32_t n = V.train_paths * V.max_levels;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        bool take = false;
-        float w = 0.f;
-        float4 r0 = {}, r1 = {}, r2 = {}, r3 = {};
-        if (i < n) {
-            uint32_t tp = i / V.max_levels, lvl = i - tp * V.max_levels;
-            if (lvl < V.n_rec[tp]) {
-                float s = 0.f;
-                for (uint32_t k = lvl + 1; k < V.max_levels; ++k) s += V.contrib[tp * V.max_levels + k];
-                const float4* r = V.rec + size_t(i) * 4;
-                r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-                w = r2.w * s;
-                take = w > 0.f && isfinite(w);
+    const uint32_t L = V.max_levels;
+    float ws = 0.f;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < V.train_paths; base += gridDim.x * blockDim.x) {
+        const uint32_t tp = base + threadIdx.x;
+        const uint32_t nr = tp < V.train_paths ? min(V.n_rec[tp], L) : 0u;
+        const float* cb = V.contrib + size_t(tp) * L;
+        const float4* rb = V.rec + size_t(tp) * L * 4;
+        uint32_t kept = 0;
+        if (nr > 0) {
+            float run = 0.f;
+            for (int lvl = int(L) - 1; lvl >= 0; --lvl) {
+                if (uint32_t(lvl) < nr) {
+                    const float w = rb[size_t(lvl) * 4 + 2].w * run;
+                    kept += (w > 0.f && isfinite(w)) ? 1u : 0u;
+                }
+                run += cb[lvl];
             }
         }
-        uint32_t pos = trd::block_append(count, take ? 1u : 0u, scan);
-        if (take) {
-            r2.w = w;
-            float4* b = batch + size_t(pos) * 4;
-            b[0] = r0, b[1] = r1, b[2] = r2, b[3] = r3;
+        uint32_t pos = trd::block_append(count, kept, scan);
+        if (kept > 0) {
+            float run = 0.f;
+            for (int lvl = int(L) - 1; lvl >= 0; --lvl) {
+                if (uint32_t(lvl) < nr) {
+                    const float4* r = rb + size_t(lvl) * 4;
+                    float4 r2 = r[2];
+                    const float w = r2.w * run;
+                    if (w > 0.f && isfinite(w)) {
+                        r2.w = w;
+                        float4* b = batch + size_t(pos) * 4;
+                        b[0] = r[0], b[1] = r[1], b[2] = r2, b[3] = r[3];
+                        ++pos;
+                        ws += w;
+                    }
+                }
+                run += cb[lvl];
+            }
         }
-        float ws = take ? w : 0.f;
-        for (int off = 32; off > 0; off >>= 1) ws += __shfl_xor(ws, off);
-        if (__lane_id() == 0) wred[threadIdx.x >> 6] = ws;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float t = 0.f;
-            for (int k = 0; k < 4; ++k) t += wred[k];
-            if (t > 0.f) atomicAdd(wsum, t);
-        }
-        __syncthreads();
+    }
+    for (int off = 32; off > 0; off >>= 1) ws += __shfl_xor(ws, off);
+    if (__lane_id() == 0) wred[threadIdx.x >> 6] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.f;
+        for (uint32_t k = 0; k < blockDim.x / 64; ++k) t += wred[k];
+        if (t > 0.f) atomicAdd(wsum, t);
     }
 }
 
@@ -667,8 +689,7 @@ int guide_wave_begin(tr_guide g, hipStream_t s)
 int guide_wave_end(tr_guide g, hipStream_t s)
 {
     if (g->view.train_paths == 0 || g->desc.steps_per_wave == 0) return TR_OK;
-    size_t n = size_t(g->view.train_paths) * g->view.max_levels;
-    uint32_t grid = uint32_t(std::min<size_t>((n + 255) / 256, 1024));
+    uint32_t grid = std::max(1u, std::min((g->view.train_paths + 255u) / 256u, 4096u));
     build_batch_kernel<<<grid, 256, 0, s>>>(g->view, g->batch, g->count, g->wsum);
     GHIP_TRY(hipGetLastError());
     return g->train(s, g->desc.steps_per_wave);
