@@ -277,7 +277,11 @@ enum {
     /* closest-hit launches run node steps and triangle steps in separate wavefront-wide phases: a
      * node's entered leaves wait until many lanes hold some (keyed 16-bit stack; BVHs of at most
      * 65,536 nodes, otherwise ignored). Launch tuning; results do not change. */
-    TR_RENDER_POSTPONE_LEAVES = 512u
+    TR_RENDER_POSTPONE_LEAVES = 512u,
+    /* camera rays (the first bounce of a single-job render) traced one query per lane like the
+     * later bounces, instead of as packets of one pixel's 64 samples per wavefront. Launch tuning
+     * and testing; results do not change. */
+    TR_RENDER_NO_CAMERA_PACKETS = 1024u
 };
 
 enum {

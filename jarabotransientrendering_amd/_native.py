@@ -30,6 +30,7 @@ TR_RENDER_STACK32 = 32
 TR_RENDER_STACK_KEYED = 64
 TR_RENDER_PER_JOB = 128
 TR_RENDER_POSTPONE_LEAVES = 512
+TR_RENDER_NO_CAMERA_PACKETS = 1024
 K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FILM_MERGE, K_FILM, K_GUIDE = 0, 1, 2, 3, 4, 5, 6
 K_NAMES = {K_CAMERA: "camera", K_EXTEND: "extend", K_SHADE: "shade", K_SHADOW: "shadow",
            K_FILM_MERGE: "film_merge", K_FILM: "film", K_GUIDE: "guide"}

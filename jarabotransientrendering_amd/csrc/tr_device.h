@@ -548,6 +548,128 @@ __device__ __forceinline__ bool trace4_step(const float4* __restrict__ nodes, co
     st.tfar = tfar;
     return done;
 }
+// ---- closest hit of a wavefront's rays as one packet ------------------------------------------
+// The camera rays of one pixel's samples nearly share their traversal. Here the wavefront visits one
+// node at a time for all its lanes: a node is visited when the ray of any active lane enters it, so
+// the node index and the traversal stack are uniform, node and triangle reads are the same address
+// for every lane, and every lane runs the same instruction stream (a per-lane query runs each node
+// step with about half the lanes of a wavefront active). Each lane keeps its own closest hit and far
+// bound, tests the triangles of a leaf its ray enters, and drops a child whose box starts beyond its
+// hit. A lane's answer is the one trace4_step gives: the closest hit does not depend on the order of
+// the visits or on visits of boxes the ray misses (equal distances resolve to the lower triangle
+// index). Children are visited nearest first by the entry distances of the first active lane.
+// `col` is the stack column of the wavefront's first lane (all lanes write the same entries).
+template <int SK>
+__device__ __forceinline__ void trace4_packet(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                              const RayPre& r, float tnear, bool active, float& tfar, int32_t& hit,
+                                              uint32_t* stack, int col, int stride)
+{
+    const uint64_t amask = __ballot(active);
+    if (amask == 0ull) return;
+    const int lead = int(__builtin_ctzll(amask));
+    auto push = [&](int sp, int32_t child) {
+        if constexpr (SK == kStack16)
+            reinterpret_cast<uint16_t*>(stack)[((sp >> 1) * stride + col) * 2 + (sp & 1)] = uint16_t(child);
+        else
+            stack[sp * stride + col] = uint32_t(child);
+    };
+    auto pop = [&](int sp) -> int32_t {
+        if constexpr (SK == kStack16)
+            return int32_t(reinterpret_cast<const uint16_t*>(stack)[((sp >> 1) * stride + col) * 2 + (sp & 1)]);
+        else
+            return int32_t(stack[sp * stride + col] & (SK == kStackKeyed ? 0xFFFFu : 0xFFFFFFFFu));
+    };
+    // the node is the same for every lane: the device pass reads it through the constant address
+    // space, so that the compiler uses scalar loads (one per wavefront, into scalar registers); the
+    // host pass, which only parses this function, has no such address space
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef __attribute__((address_space(4))) const uint4 cuint4;
+#else
+    typedef const uint4 cuint4;
+#endif
+    int32_t node = 0;
+    int sp = 0;
+    for (;;) {
+        node = __builtin_amdgcn_readfirstlane(node);
+        const cuint4* n = (const cuint4*)(reinterpret_cast<const uint4*>(nodes) + 4 * size_t(node));
+        const uint4 w0 = n[0], w1 = n[1], w2 = n[2], w3 = n[3];
+        const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+        const float sx = __uint_as_float(w2.z), sy = __uint_as_float(w2.w), sz = __uint_as_float(w0.w);
+        const uint32_t ref[4] = {w3.x, w3.y, w3.z, w3.w};
+        int cha[4], cna[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            cha[k] = int(ref[k] & 0x1FFFFFFFu);
+            const int c = int(ref[k] >> 29);
+            cna[k] = c == int(kQEmptyDev) ? -1 : c;
+        }
+        // the slab test of trace4_step (same planes, same margins)
+        const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
+        const float bx = fmaf(ox, r.inv.x, r.c.x), by = fmaf(oy, r.inv.y, r.c.y), bz = fmaf(oz, r.inv.z, r.c.z);
+        const float bx0 = fmaf(-r.k, fabsf(r.inv.x), bx), bx1 = fmaf(r.k, fabsf(r.inv.x), bx);
+        const float by0 = fmaf(-r.k, fabsf(r.inv.y), by), by1 = fmaf(r.k, fabsf(r.inv.y), by);
+        const float bz0 = fmaf(-r.k, fabsf(r.inv.z), bz), bz1 = fmaf(r.k, fabsf(r.inv.z), bz);
+        const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
+        const uint32_t nx = px ? w1.x : w1.y, fx = px ? w1.y : w1.x;
+        const uint32_t ny = py ? w1.z : w1.w, fy = py ? w1.w : w1.z;
+        const uint32_t nz = pz ? w2.x : w2.y, fz = pz ? w2.y : w2.x;
+        float key[4];
+        bool ent[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float x0 = fmaf(float((nx >> (8 * k)) & 0xFFu), ax, bx0), x1 = fmaf(float((fx >> (8 * k)) & 0xFFu), ax, bx1);
+            const float y0 = fmaf(float((ny >> (8 * k)) & 0xFFu), ay, by0), y1 = fmaf(float((fy >> (8 * k)) & 0xFFu), ay, by1);
+            const float z0 = fmaf(float((nz >> (8 * k)) & 0xFFu), az, bz0), z1 = fmaf(float((fz >> (8 * k)) & 0xFFu), az, bz1);
+            const float t0 = fmaxf(fmaxf(x0, y0), fmaxf(z0, tnear));
+            const float t1 = fminf(fminf(x1, y1), fminf(z1, tfar)) * 1.00000036f;
+            ent[k] = active && t0 <= t1 && cna[k] >= 0;
+            key[k] = t0;
+        }
+        // leaves first: the wavefront loops over the triangles of every leaf some lane entered;
+        // each lane tests those of the leaves its ray entered
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (cna[k] > 0 && __ballot(ent[k]) != 0ull) {
+                const int i0 = __builtin_amdgcn_readfirstlane(cha[k]), i1 = i0 + __builtin_amdgcn_readfirstlane(cna[k]);
+                for (int i = i0; i < i1; ++i)
+                    if (ent[k]) tri_hit(r, reinterpret_cast<const float*>(tris + 4 * size_t(i)), tnear, tfar, hit);
+            }
+        }
+        // inner children that some lane enters before its hit, ordered by the first active lane's
+        // entry distances (children that lane does not enter last)
+        float uk[4];
+        int ui[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool e = ent[k] && cna[k] == 0 && key[k] <= tfar * 1.00000036f;
+            const bool any = __ballot(e) != 0ull;
+            const float lk = __shfl(e ? key[k] : __int_as_float(0x7f800000), lead);
+            uk[k] = any ? lk : __int_as_float(0x7f800000);
+            ui[k] = any ? __builtin_amdgcn_readfirstlane(cha[k]) : -1;
+        }
+#define TR_PSWAP(a, b)                                                                \
+    if (ui[b] >= 0 && (ui[a] < 0 || uk[b] < uk[a])) {                                 \
+        float tk = uk[a]; uk[a] = uk[b]; uk[b] = tk;                                  \
+        int ti = ui[a]; ui[a] = ui[b]; ui[b] = ti;                                    \
+    }
+        TR_PSWAP(0, 1) TR_PSWAP(2, 3) TR_PSWAP(0, 2) TR_PSWAP(1, 3) TR_PSWAP(1, 2)
+#undef TR_PSWAP
+        if (ui[0] < 0) {
+            if (sp == 0) break;
+            --sp;
+            node = pop(sp);
+            continue;
+        }
+#pragma unroll
+        for (int k = 3; k >= 1; --k)
+            if (ui[k] >= 0) {
+                push(sp, ui[k]);
+                ++sp;
+            }
+        node = ui[0];
+    }
+}
+
 // ---- closest hit with postponed leaves -------------------------------------------------------
 // A closest-hit query whose wavefront runs node steps and triangle steps as separate, wavefront-wide
 // phases: a node step records the node's entered leaves as the lane's pending leaves instead of

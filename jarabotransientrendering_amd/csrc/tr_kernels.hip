@@ -495,6 +495,47 @@ __global__ void __launch_bounds__(kTraceBlock, 7) extend_kernel(SceneDev S0, con
     }
 }
 
+// extend for the camera rays of single-job renders (the first bounce): the queue holds them in path
+// order, so a wavefront's 64 consecutive entries are samples of one pixel, traced as one packet
+// (trace4_packet); the lanes finish together and the refill hands them the next 64 entries. The
+// hits are those of extend_kernel (the flag TR_RENDER_NO_CAMERA_PACKETS selects it instead).
+template <int TRAV, int STACK>
+__global__ void __launch_bounds__(kTraceBlock, 7) extend_camera_kernel(SceneDev S0, WaveDev Wv, int cur)
+{
+    static_assert(TRAV == kTravWide || TRAV == kTravWide16 || TRAV == kTravWideKeyed, "4-wide stacks");
+    __shared__ uint32_t stack[stack_words<TRAV, STACK>() * kTraceBlock];
+    const uint32_t count = Wv.counters[qword(cur)];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0) {
+        Wv.counters[qword(cur ^ 1)] = 0;
+        Wv.counters[shword(cur ^ 1)] = 0;
+        atomicAdd(reinterpret_cast<unsigned long long*>(Wv.counters + C_STAT_EXT), (unsigned long long)count);
+    }
+    if (gid < kParts) Wv.counters[part_word(gid, P_FETCH_SHADOW)] = 0;
+    const uint32_t chunk = fetch_chunk(count);
+    RayRefill rf(count, chunk);
+    bool busy = false;
+    uint32_t rec = 0;
+    RayPre r;
+    const int col = int(threadIdx.x & ~63u);
+    const f3 cam = mk(Wv.cpos.x, Wv.cpos.y, Wv.cpos.z);
+    for (;;) {
+        rf.run(count, chunk, Wv.counters, P_FETCH_EXT, busy, [&](uint32_t i) {
+            rec = i;
+            r = ray_pre(cam, mk(Wv.ray_d[cur][i]), S0.plane_bound);
+        });
+        if (__ballot(busy) == 0ull) break;  // the queue is drained
+        float tfar = __int_as_float(0x7f800000);
+        int32_t hit = -1;
+        trace4_packet<stack_kind<TRAV>()>(S0.nodes4, S0.tris, r, kSigmaVisibility, busy, tfar, hit, stack, col,
+                                          kTraceBlock);
+        if (busy) {
+            Wv.hit[rec] = make_float2(hit >= 0 ? tfar : -1.0f, __int_as_float(hit));
+            busy = false;
+        }
+    }
+}
+
 // Lambertian cosine-weighted direction in the reference's shading frame
 // (Sampling.h:131-143, Intersection.h:167-178, Lambertian.h:107-122).
 __device__ __forceinline__ f3 sample_lambert(f3 n, float e1, float e2, float* cos_sampled)
@@ -1376,6 +1417,14 @@ void launch_timed(void (*k)(P...), uint32_t grid, uint32_t block, hipStream_t s,
         default: TR_STACK_DISPATCH3B(kTravWideKeyed, bound, batch, KERNEL, GRID, BLOCK, STREAM, __VA_ARGS__); break;       \
         }                                                                                                   \
     } while (0)
+// camera-ray packets (extend_camera_kernel), timed like the other extend launches
+#define TR_CAMERA_DISPATCH_T(T, bound, GRID, BLOCK, STREAM, ...)                                                      \
+    do {                                                                                                             \
+        if ((bound) < 16) launch_timed(extend_camera_kernel<T, 16>, GRID, BLOCK, STREAM, ev_start, ev_stop, __VA_ARGS__); \
+        else if ((bound) < 24) launch_timed(extend_camera_kernel<T, 24>, GRID, BLOCK, STREAM, ev_start, ev_stop, __VA_ARGS__); \
+        else if ((bound) < 32) launch_timed(extend_camera_kernel<T, 32>, GRID, BLOCK, STREAM, ev_start, ev_stop, __VA_ARGS__); \
+        else launch_timed(extend_camera_kernel<T, kBvhStack>, GRID, BLOCK, STREAM, ev_start, ev_stop, __VA_ARGS__);      \
+    } while (0)
 #define TR_STACK_DISPATCH_T(T, bound, KERNEL, GRID, BLOCK, STREAM, ...)                   \
     do {                                                                                   \
         if ((bound) < 16) KERNEL<T, 16><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__);          \
@@ -1463,6 +1512,8 @@ struct LaunchCfg {
     // batches: render the jobs one after another (single-job kernels, LDS-privatised film) instead of
     // in one shared sequence of wavefronts
     bool per_job = false;
+    // single-job renders: camera rays traced as per-pixel packets (extend_camera_kernel)
+    bool camera_packets = true;
 };
 
 // launch flags -> stack format
@@ -2619,8 +2670,16 @@ struct Launch {
         int cur = 0;
         for (int b = 0; b < bounces; ++b) {
             if ((rc = begin(TR_K_EXTEND))) return rc;
-            TR_TRACE_DISPATCH(trav_ext, stack_bound, batch, extend_kernel, tg, kTraceBlock, stream, S0, dj, Wv,
-                              cur | (b == 0 ? 2 : 0));
+            if (b == 0 && !batch && c.camera_packets &&
+                (trav_ext == kTravWide || trav_ext == kTravWide16 || trav_ext == kTravWideKeyed)) {
+                if (trav_ext == kTravWide) TR_CAMERA_DISPATCH_T(kTravWide, stack_bound, tg, kTraceBlock, stream, S0, Wv, cur);
+                else if (trav_ext == kTravWide16)
+                    TR_CAMERA_DISPATCH_T(kTravWide16, stack_bound, tg, kTraceBlock, stream, S0, Wv, cur);
+                else TR_CAMERA_DISPATCH_T(kTravWideKeyed, stack_bound, tg, kTraceBlock, stream, S0, Wv, cur);
+            } else {
+                TR_TRACE_DISPATCH(trav_ext, stack_bound, batch, extend_kernel, tg, kTraceBlock, stream, S0, dj, Wv,
+                                  cur | (b == 0 ? 2 : 0));
+            }
             if ((rc = end())) return rc;
             if ((rc = begin(TR_K_SHADE))) return rc;
             if (batch) {
@@ -2889,6 +2948,7 @@ int render_impl(tr_scene s, const tr_camera* cam, tr_film f, const tr_render_des
     c.prof_extend = (d->flags & TR_RENDER_PROFILE) == 0 && (d->flags & TR_RENDER_PROFILE_EXTEND) != 0;
     if (d->flags & (TR_RENDER_STACK32 | TR_RENDER_STACK_KEYED | TR_RENDER_POSTPONE_LEAVES)) c.stack_fmt = stack_fmt_of(d->flags);
     c.mode = guide ? kShadeGuided : ((d->flags & TR_RENDER_ORTHONORMAL_FRAME) ? kShadeOnb : kShadeRef);
+    c.camera_packets = (d->flags & TR_RENDER_NO_CAMERA_PACKETS) == 0;
     return run_jobs(s->stream, s->wave, jobs, c, guide, s->compute_units, s->ev, &s->ev_made, stats, t_start);
 }
 

@@ -53,7 +53,8 @@ def test_python_constants_match_the_header():
         if hasattr(nat, name):
             assert getattr(nat, name) == val, name
     for name in ("TR_RENDER_PROFILE", "TR_RENDER_PROFILE_EXTEND", "TR_RENDER_PER_JOB", "TR_RENDER_TIME_SAMPLING",
-                 "TR_RENDER_STACK32", "TR_RENDER_STACK_KEYED", "TR_RENDER_GRAPH"):
+                 "TR_RENDER_STACK32", "TR_RENDER_STACK_KEYED", "TR_RENDER_GRAPH", "TR_RENDER_POSTPONE_LEAVES",
+                 "TR_RENDER_NO_CAMERA_PACKETS"):
         assert name in values and getattr(nat, name) == values[name]
 
 

@@ -384,11 +384,14 @@ def test_binary_and_wide_bvh_render_the_same(R):
     np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-7)
 
 
-@pytest.mark.parametrize("flag", ["TR_RENDER_STACK32", "TR_RENDER_STACK_KEYED", "TR_RENDER_POSTPONE_LEAVES"])
+@pytest.mark.parametrize("flag", ["TR_RENDER_STACK32", "TR_RENDER_STACK_KEYED", "TR_RENDER_POSTPONE_LEAVES",
+                                  "TR_RENDER_NO_CAMERA_PACKETS"])
 def test_traversal_launch_formats_render_the_same(R, flag):
-    """The closest-hit traversal variants (32-bit stack, keyed stack, postponed leaves) find the
-    same hits as the default (16-bit stack, leaves tested in the node step): same extension and
-    shadow ray counts, and the same film up to float summation order."""
+    """The closest-hit traversal variants (32-bit stack, keyed stack, postponed leaves, camera rays
+    one query per lane instead of per-pixel packets) find the same hits as the default (16-bit
+    stack, leaves tested in the node step, camera-ray packets): same extension and shadow ray
+    counts, and the same film up to float summation order. At 9 samples per pixel a packet of 64
+    camera rays spans seven pixels, the packets' least coherent case."""
     from jarabotransientrendering_amd import _native as nat
     s = scenes.icosphere_box(level=5, width=16, height=16, time_bins=256, sqrt_spp=3, n_incoming=2, max_bounces=6,
                              seed=23)
