@@ -1,10 +1,12 @@
-"""The traversal's ray-space slab test (csrc/tr_device.h, trace4_step) evaluates each quantized
-plane as one fused multiply-add, q * (step * inv) + (origin - o) * inv, and widens the entry and exit
-distances of each axis by e = 3e-7 ((255 step + |origin|) |inv| + |b|). This checks, on the CPU with
-float32 arithmetic, that the widened interval always contains the exact distances of the planes as
-the host rounds them (float32(origin + q * step)), so the box test stays conservative; and that the
-bound is not loose by orders of magnitude (the widening stays a few float32 ulps of the slab's
-distances scale)."""
+"""The traversal's ray-space slab test (csrc/tr_device.h, trace4_step / trace4_packet, offsets and
+margin from ray_pre) evaluates each quantized plane as two fused multiply-adds: b = fma(origin, inv, c)
+with the ray's c = -(o * inv), then fma(q, step * inv, b -+ k |inv|), where b -+ k |inv| is itself a
+fused multiply-add and k = 6e-7 (C + max |o|), C bounding |origin| + 255 step over all nodes
+(qbvh_plane_bound, rounded up). This checks, on the CPU with float32 arithmetic, that the widened
+interval always contains the exact distances of the planes as the host rounds them
+(float32(origin + q * step)), so the box test stays conservative, with the tightest valid C (the
+node's own |origin| + 255 step) and the ray's own |o| (the smallest margin the kernels can use); and
+that the widening stays a small multiple of the slab's rounding scale."""
 import numpy as np
 
 F = np.float32
@@ -30,24 +32,32 @@ def _cases(n, rng):
     return origin, step, q, o, inv
 
 
+def _margin(origin, step, o):
+    # the device's k = 6e-7f * (plane_bound + max|o|), with the smallest valid plane bound
+    C = np.nextafter((np.abs(origin).astype(np.float64) + 255.0 * step.astype(np.float64)).astype(F), F(np.inf))
+    return (F(6e-7) * (C + np.abs(o)).astype(F)).astype(F), C
+
+
 def test_widened_slab_contains_exact_plane_distance():
     rng = np.random.default_rng(5)
     origin, step, q, o, inv = _cases(400_000, rng)
     plane = (origin.astype(np.float64) + q.astype(np.float64) * step.astype(np.float64)).astype(F)  # host rounding
     exact = (plane.astype(np.float64) - o.astype(np.float64)) * inv.astype(np.float64)
     a = (step * inv).astype(F)
-    b = ((origin - o).astype(F) * inv).astype(F)
-    e = (F(3e-7) * _fma32(_fma32(np.full_like(step, 255.0), step, np.abs(origin)), np.abs(inv), np.abs(b))).astype(F)
-    entry = _fma32(q, a, (b - e).astype(F))
-    exit_ = _fma32(q, a, (b + e).astype(F))
+    c = (-(o * inv).astype(F)).astype(F)
+    k, C = _margin(origin, step, o)
+    b = _fma32(origin, inv, c)
+    ai = np.abs(inv).astype(F)
+    entry = _fma32(q, a, _fma32(-k, ai, b))
+    exit_ = _fma32(q, a, _fma32(k, ai, b))
     # one float32 ulp of slack for the emulated fused multiply-add's double rounding
     ulp = np.spacing(np.abs(entry)).astype(np.float64)
     assert (entry.astype(np.float64) - ulp <= exact).all()
     ulp = np.spacing(np.abs(exit_)).astype(np.float64)
     assert (exit_.astype(np.float64) + ulp >= exact).all()
     # the widening is a small multiple of the rounding scale of the slab's terms
-    scale = (255.0 * step.astype(np.float64) + np.abs(origin)) * np.abs(inv) + np.abs(b)
-    assert (e.astype(np.float64) <= 3.1e-7 * scale).all()
+    scale = (C.astype(np.float64) + np.abs(o)) * np.abs(inv)
+    assert (k.astype(np.float64) * np.abs(inv) <= 6.1e-7 * scale).all()
 
 
 def test_a_dropped_widening_fails_the_check():
@@ -58,7 +68,25 @@ def test_a_dropped_widening_fails_the_check():
     plane = (origin.astype(np.float64) + q.astype(np.float64) * step.astype(np.float64)).astype(F)
     exact = (plane.astype(np.float64) - o.astype(np.float64)) * inv.astype(np.float64)
     a = (step * inv).astype(F)
-    b = ((origin - o).astype(F) * inv).astype(F)
-    t = _fma32(q, a, b).astype(np.float64)
+    c = (-(o * inv).astype(F)).astype(F)
+    t = _fma32(q, a, _fma32(origin, inv, c)).astype(np.float64)
     # unwidened, an entry distance can lie beyond the exact one (a box missed) and an exit before it
     assert (t > exact).sum() > 100 and (t < exact).sum() > 100
+
+
+def test_a_margin_too_small_fails_the_check():
+    """A margin of a tenth of the kernels' (k = 6e-8 (C + |o|)) is not enough on these cases, so the
+    margin's size is what the first check pins."""
+    rng = np.random.default_rng(7)
+    origin, step, q, o, inv = _cases(400_000, rng)
+    plane = (origin.astype(np.float64) + q.astype(np.float64) * step.astype(np.float64)).astype(F)
+    exact = (plane.astype(np.float64) - o.astype(np.float64)) * inv.astype(np.float64)
+    a = (step * inv).astype(F)
+    c = (-(o * inv).astype(F)).astype(F)
+    k, _ = _margin(origin, step, o)
+    k = (k * F(0.1)).astype(F)
+    b = _fma32(origin, inv, c)
+    ai = np.abs(inv).astype(F)
+    entry = _fma32(q, a, _fma32(-k, ai, b)).astype(np.float64)
+    exit_ = _fma32(q, a, _fma32(k, ai, b)).astype(np.float64)
+    assert (entry > exact).sum() + (exit_ < exact).sum() > 0
