@@ -297,7 +297,10 @@ constexpr int stack_kind() { return TRAV == kTravWide ? kStack32 : TRAV == kTrav
 // claims `chunk` queue entries at a time (its static share first, then through its partition's
 // fetch cursor) and hands them to its idle lanes in lane order. Call with every lane of the
 // wavefront; `busy` marks the lanes with a query.
-constexpr int kRefillMin = 24;  // idle lanes that trigger a refill (unless the whole wavefront is idle)
+// idle lanes that trigger a refill (unless the whole wavefront is idle): closest-hit and any-hit launches
+// (with camera rays in their own packet kernel, 16 and 32 for either were 0.2-0.7 % slower than 24)
+constexpr int kRefillMinExtend = 24;
+constexpr int kRefillMinShadow = 24;
 // Share of its even split of the queue that each wavefront takes without an atomic, in quarters
 // (0: one fetch chunk). Larger static shares measured slower on the 82k-triangle scene, whose rays'
 // costs vary across the image, and faster on the 22-triangle media scene, whose launches the
@@ -336,7 +339,7 @@ struct RayRefill {
 
     // Assigns queue entries to idle lanes; `start(i)` begins the query of entry i on this lane.
     // `cursor` is the partition word (P_FETCH_EXT / P_FETCH_SHADOW) of the launch's queue.
-    template <class Start>
+    template <int kRefillMin = kRefillMinExtend, class Start>
     __device__ __forceinline__ void run(uint32_t count, uint32_t chunk, uint32_t* counters, uint32_t cursor, bool& busy,
                                         Start start)
     {
@@ -1062,7 +1065,7 @@ __global__ void __launch_bounds__(kTraceBlock, 7) shadow_kernel(SceneDev S0, con
         Trav4 st;
         const float4 *nodes = S0.nodes4, *tris = S0.tris;
         for (;;) {
-            rf.run(count, chunk, Wv.counters, P_FETCH_SHADOW, busy, [&](uint32_t i) {
+            rf.run<kRefillMinShadow>(count, chunk, Wv.counters, P_FETCH_SHADOW, busy, [&](uint32_t i) {
                 rec = i;
                 float pb = S0.plane_bound;
                 if constexpr (BATCH) {
