@@ -739,7 +739,8 @@ def main():
             progress(f"guided comparison: {k}")
             kw = dict(v)
             if quick:
-                kw["repeats"] = 1
+                # three trained guides, median reported: one guide in eight trains a weak network
+                kw["repeats"] = 3
                 kw["ref_spp"] = kw["ref_spp"] // 4
             res[k] = guided_comparison(k, **kw)
         # the headline half of the metric: time to the plain tracer's relMSE, guided vs plain, on the
